@@ -1,0 +1,864 @@
+// rd_oracle.cpp -- CPU oracle for the ridge-detection choose / evolve / decimate path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain C++ restatement of the reference's
+// serial CPU implementation (aosewski/RidgeDetection, `ridge_detection/rd/cpu/...`,
+// built without RD_USE_OPENMP).  It is the checker for the HIP library in
+// `ridgedetection_amd/csrc`: only `tests/`, `__graft_entry__.smoke()` and the
+// `cpu_baseline` leg of `bench.py` may load it, and never as the thing measured or
+// shipped.  Every function names the reference file:line it follows; floating-point
+// operations are written in the same order as the reference so that results are
+// bit-identical on x86-64 (no FMA contraction: build with -ffp-contract=off).
+//
+// Pinning (what checks each function, beyond retyping the formula):
+//   * golden fixtures produced by the compiled reference headers (oracle/ref_driver.cpp,
+//     tests/golden/make_golden.py) for choose, one evolve iteration, full evolve,
+//     decimate, brute-force ridge detection, the three tile trees and the five tiled
+//     drivers (tests/test_oracle_golden.py);
+//   * mathematical invariants (greedy cover / separation for choose, brute-force
+//     argmin for the evolve assignment, partition invariants for the trees)
+//     in tests/test_oracle_invariants.py.
+//
+// Conventions: points are ROW_MAJOR (point i occupies P[i*dim .. i*dim+dim-1]).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+
+// ---------------------------------------------------------------------------------
+// a1: squared Euclidean distance, accumulated from 0 in dimension order.
+//     rd/cpu/brute_force/rd_inner.hpp:41-49
+// ---------------------------------------------------------------------------------
+template <typename T>
+static inline T sqdist(const T* a, const T* b, int dim) {
+    T acc = 0;
+    for (int d = 0; d < dim; ++d) {
+        T t = a[d] - b[d];
+        acc += t * t;
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------------
+// a3: greedy choose.  rd/cpu/brute_force/choose.hpp:36-70 (+ rd_inner.hpp:66-86)
+// Point 0 is always chosen; point i is chosen iff no already chosen point lies
+// within squared distance <= r*r.  Returns K; writes coordinates and input indices.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static long choose(const T* P, const long* order, long n, int dim, T r, std::vector<T>& S,
+                   std::vector<long>* chosenIdx) {
+    const T rSqr = r * r;
+    S.clear();
+    if (chosenIdx) chosenIdx->clear();
+    if (n < 1) return 0;
+    auto pt = [&](long i) { return P + (order ? order[i] : i) * (long)dim; };
+    S.insert(S.end(), pt(0), pt(0) + dim);
+    if (chosenIdx) chosenIdx->push_back(order ? order[0] : 0);
+    long count = 1;
+    for (long i = 1; i < n; ++i) {
+        const T* p = pt(i);
+        bool found = false;
+        const T* s = S.data();
+        for (long k = 0; k < count; ++k, s += dim) {
+            if (sqdist(p, s, dim) <= rSqr) { found = true; break; }
+        }
+        if (!found) {
+            S.insert(S.end(), p, p + dim);
+            if (chosenIdx) chosenIdx->push_back(order ? order[i] : i);
+            ++count;
+        }
+    }
+    return count;
+}
+
+// ---------------------------------------------------------------------------------
+// a4: nearest chosen point + r-ball masked coordinate sums, for one block of samples.
+//     rd/cpu/brute_force/evolve.hpp:32-77.  Members are summed in sample order.
+//     labels (optional) receives the chosen index of every member sample, or -1.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static void closestSphereCenter(const T* P, const long* idx, long n, const T* S, long K, int dim,
+                                T rSqr, T* sums, int* counts, int* labels) {
+    for (long i = 0; i < n; ++i) {
+        const T* p = P + (idx ? idx[i] : i) * (long)dim;
+        const T* s = S;
+        T minSq = std::numeric_limits<T>::max();
+        int minK = -1;
+        for (long k = 0; k < K; ++k) {
+            T sq = 0;
+            for (int d = 0; d < dim; ++d, ++s) {
+                T t = *s - p[d];
+                sq += t * t;
+            }
+            if (sq < minSq) { minK = (int)k; minSq = sq; }
+        }
+        if (minSq <= rSqr) {
+            counts[minK]++;
+            for (int d = 0; d < dim; ++d) sums[(long)minK * dim + d] += p[d];
+            if (labels) labels[i] = minK;
+        } else if (labels) {
+            labels[i] = -1;
+        }
+    }
+}
+
+// OpenMP form of the same pass (rd/cpu/brute_force/evolve.hpp:79-127): members are
+// accumulated inside a critical section, so the summation order is not deterministic.
+// Used only to time the multi-threaded CPU baseline.
+template <typename T>
+static void closestSphereCenterOmp(const T* P, long n, const T* S, long K, int dim, T rSqr,
+                                   T* sums, int* counts) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const T* p = P + i * (long)dim;
+        const T* s = S;
+        T minSq = std::numeric_limits<T>::max();
+        int minK = -1;
+        for (long k = 0; k < K; ++k) {
+            T sq = 0;
+            for (int d = 0; d < dim; ++d, ++s) {
+                T t = *s - p[d];
+                sq += t * t;
+            }
+            if (sq < minSq) { minK = (int)k; minSq = sq; }
+        }
+        if (minSq <= rSqr) {
+#pragma omp critical
+            {
+                counts[minK]++;
+                for (int d = 0; d < dim; ++d) sums[(long)minK * dim + d] += p[d];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// a5: shift toward the mass centre.  rd/cpu/brute_force/evolve.hpp:129-179.
+// Only the coordinates whose move is "numerically distinguishable" are updated.
+// threshold = ((2 * |mc|) * count) * eps, evaluated in T.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static int shiftTowardMassCenter(T* S, const T* sums, const int* counts, long K, int dim,
+                                 unsigned char* movedMask) {
+    int cont = 0;
+    const T eps = std::numeric_limits<T>::epsilon();
+    for (long k = 0; k < K; ++k) {
+        for (int d = 0; d < dim; ++d) {
+            unsigned char moved = 0;
+            if ((T)counts[k]) {
+                T mc = sums[k * dim + d] / (T)counts[k];
+                T thr = T(2) * std::fabs(mc);
+                thr = thr * (T)counts[k];
+                thr = thr * eps;
+                if (std::fabs(mc - S[k * dim + d]) > thr) {
+                    cont = 1;
+                    S[k * dim + d] = mc;
+                    moved = 1;
+                }
+            }
+            if (movedMask) movedMask[k * dim + d] = moved;
+        }
+    }
+    return cont;
+}
+
+// One evolve iteration over a list of sample blocks (evolve.hpp:313-338 / 367-395).
+template <typename T>
+static int evolveIteration(const T* P, const std::vector<std::pair<const long*, long>>& blocks,
+                           T* S, long K, int dim, T r, T* sums, int* counts, int* labels,
+                           unsigned char* movedMask) {
+    const T rSqr = r * r;
+    std::fill(sums, sums + K * dim, T(0));
+    std::fill(counts, counts + K, 0);
+    long off = 0;
+    for (auto& b : blocks) {
+        closestSphereCenter(P, b.first, b.second, S, K, dim, rSqr, sums, counts,
+                            labels ? labels + off : nullptr);
+        off += b.second;
+    }
+    return shiftTowardMassCenter(S, sums, counts, K, dim, movedMask);
+}
+
+// a6: evolve until no coordinate moves (evolve.hpp:292-342, 344-399).  maxIter caps the
+// number of iterations (0 = unlimited, the reference behaviour).  Returns iterations.
+template <typename T>
+static long evolve(const T* P, const std::vector<std::pair<const long*, long>>& blocks, T* S,
+                   long K, int dim, T r, long maxIter) {
+    std::vector<T> sums((size_t)std::max<long>(K, 1) * dim);
+    std::vector<int> counts((size_t)std::max<long>(K, 1));
+    long it = 0;
+    int cont = 1;
+    while (cont) {
+        cont = evolveIteration(P, blocks, S, K, dim, r, sums.data(), counts.data(), nullptr, nullptr);
+        ++it;
+        if (maxIter > 0 && it >= maxIter) break;
+    }
+    return it;
+}
+
+// ---------------------------------------------------------------------------------
+// a7: decimate.  rd/cpu/brute_force/decimate.hpp:40-81.  A point is removed when it has
+// >= 4 points (itself included) within r2, or < 3 within 2*r2 (squared: 4*r2^2).
+// Removal is immediate; the pass stops when fewer than 3 points remain; passes repeat
+// until nothing changes or at most 3 points are left.  `alive` flags are updated.
+//
+// a8: decimateNoCopy, rd/cpu/brute_force/decimate.hpp:84-127: same test, but a point is
+// kept (and the pass continues) when removing it would leave fewer than 3 points.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static bool hasAtLeast(const std::vector<const T*>& pts, const std::vector<unsigned char>& alive,
+                       const T* src, int dim, T rSqr, int threshold) {
+    for (size_t j = 0; j < pts.size(); ++j) {
+        if (!alive[j]) continue;
+        if (sqdist(src, pts[j], dim) <= rSqr) {
+            if (--threshold <= 0) return true;
+        }
+    }
+    return false;
+}
+
+template <typename T>
+static long decimateImpl(const std::vector<const T*>& pts, std::vector<unsigned char>& alive,
+                         long cnt, int dim, T r, bool noCopyRule) {
+    const T rSqr = r * r;
+    const T r4 = T(4) * rSqr;
+    long left = 0;
+    while (left != cnt && cnt > 3) {
+        left = cnt;
+        for (size_t i = 0; i < pts.size(); ++i) {
+            if (!alive[i]) continue;
+            bool rm = hasAtLeast(pts, alive, pts[i], dim, rSqr, 4) ||
+                      !hasAtLeast(pts, alive, pts[i], dim, r4, 3);
+            if (!rm) continue;
+            if (noCopyRule) {
+                if (cnt - 1 < 3) continue;
+                --cnt;
+                alive[i] = 0;
+            } else {
+                --cnt;
+                alive[i] = 0;
+                if (cnt < 3) break;
+            }
+        }
+    }
+    return cnt;
+}
+
+// ---------------------------------------------------------------------------------
+// a9: brute-force ridge detection.  rd/cpu/brute_force/ridge_detection.inl:67-146 and
+// the sample-list overload :171-263 (choose over the first block only, evolve over all).
+// ---------------------------------------------------------------------------------
+template <typename T>
+static long ridgeDetection(const T* P, const std::vector<std::pair<const long*, long>>& blocks,
+                           int dim, T r1, T r2, long maxEvolveIter, std::vector<T>& S,
+                           long* roundsOut) {
+    long K = choose(P, blocks[0].first, blocks[0].second, dim, r1, S, nullptr);
+    long oldCount = 0;
+    long rounds = 0;
+    while (oldCount != K) {
+        oldCount = K;
+        evolve(P, blocks, S.data(), K, dim, r1, maxEvolveIter);
+        std::vector<const T*> pts((size_t)K);
+        for (long k = 0; k < K; ++k) pts[k] = S.data() + k * dim;
+        std::vector<unsigned char> alive((size_t)K, 1);
+        long nk = decimateImpl(pts, alive, K, dim, r2, false);
+        std::vector<T> ns;
+        ns.reserve((size_t)nk * dim);
+        for (long k = 0; k < K; ++k)
+            if (alive[k]) ns.insert(ns.end(), S.begin() + k * dim, S.begin() + (k + 1) * dim);
+        S.swap(ns);
+        K = nk;
+        ++rounds;
+    }
+    if (roundsOut) *roundsOut = rounds;
+    return K;
+}
+
+// ---------------------------------------------------------------------------------
+// a10: bounding box.  rd/utils/bounding_box.hpp:127-227, 314-418.
+// ---------------------------------------------------------------------------------
+template <typename T>
+struct Box {
+    int dim = 0;
+    std::vector<T> mn, mx, dist;
+
+    void tight(const T* P, const long* idx, long n, int aDim) {
+        dim = aDim;
+        mn.assign(dim, std::numeric_limits<T>::max());
+        mx.assign(dim, std::numeric_limits<T>::lowest());
+        for (long k = 0; k < n; ++k) {
+            const T* p = P + idx[k] * (long)dim;
+            for (int i = 0; i < dim; ++i) {
+                T v = p[i];
+                mn[i] = (v < mn[i]) ? v : mn[i];
+                mx[i] = (v > mx[i]) ? v : mx[i];
+            }
+        }
+        calcDistances();
+    }
+    void calcDistances() {
+        dist.assign(dim, T(0));
+        for (int k = 0; k < dim; ++k) {
+            dist[k] = std::abs(mx[k] - mn[k]);
+            if (dist[k] <= std::numeric_limits<T>::epsilon()) dist[k] = 0;
+        }
+    }
+    size_t countSpheresInside(T radius) const {
+        size_t cnt = 1;
+        for (int d = 0; d < dim; ++d) {
+            size_t dc = static_cast<size_t>(std::ceil(dist[d] / radius));
+            cnt *= dc ? dc + 1 : 1;
+        }
+        return cnt;
+    }
+    bool isInside(const T* p) const {
+        for (int d = 0; d < dim; ++d)
+            if (p[d] < mn[d] || p[d] >= mx[d]) return false;
+        return true;
+    }
+    bool isNearby(const T* p, T r) const {
+        for (int d = 0; d < dim; ++d)
+            if ((p[d] < mn[d] - r) || (p[d] >= mx[d] + r)) return false;
+        return !isInside(p);
+    }
+    bool overlap(const Box& o) const {
+        if (dim != o.dim) return false;
+        for (int d = 0; d < dim; ++d) {
+            bool res = false;
+            res = res || (o.mn[d] >= mn[d] && o.mn[d] <= mx[d]);
+            res = res || (o.mx[d] >= mn[d] && o.mx[d] <= mx[d]);
+            res = res || (mn[d] >= o.mn[d] && mn[d] <= o.mx[d]);
+            res = res || (mx[d] >= o.mn[d] && mx[d] <= o.mx[d]);
+            if (!res) return false;
+        }
+        return true;
+    }
+};
+
+// a11: histogram bin of a point.  rd/cpu/tiled/local_tile_tree.inl:195-276.
+// Dimension 0 varies fastest in the linear id.  An id beyond the histogram (possible
+// in the reference only through rounding, where it is undefined behaviour) is clamped.
+template <typename T>
+static size_t binOf(const T* x, const std::vector<size_t>& nb, const Box<T>& bb) {
+    const T eps = std::numeric_limits<T>::epsilon();
+    size_t idx = 0, mul = 1;
+    for (int i = 0; i < bb.dim; ++i) {
+        size_t b = 0;
+        if (!(bb.dist[i] < eps)) {
+            T normCord = std::abs(x[i] - bb.mn[i]);
+            T step = bb.dist[i] / T(nb[i]);
+            if (std::abs(normCord - bb.dist[i]) <= eps || normCord - bb.dist[i] > 0)
+                b = nb[i] - 1;
+            else
+                b = (size_t)std::floor(normCord / step);
+        }
+        idx += b * mul;
+        mul *= nb[i];
+    }
+    size_t total = 1;
+    for (size_t v : nb) total *= v;
+    return idx < total ? idx : total - 1;
+}
+
+// ---------------------------------------------------------------------------------
+// a12/a13: tile trees.
+//   LOCAL    -- rd/cpu/tiled/local_tile_tree.inl:48-437   (TiledLocalTree)
+//   GROUPED  -- rd/cpu/tiled/grouped_tile_tree.inl:86-415, 517-624 (FIND_NEIGHBOURS)
+//   EXTENDED -- rd/cpu/tiled/grouped_tile_tree.inl:86-415, 423-510 (EXTENDED_TILE)
+// Tiles hold input indices instead of copied coordinates; order is the reference's
+// copy order.  Leaves are listed in preorder (collectLeafs).
+// ---------------------------------------------------------------------------------
+enum TreeKind { TREE_LOCAL = 0, TREE_GROUPED = 1, TREE_EXTENDED = 2 };
+
+template <typename T>
+struct Tile {
+    std::vector<long> own;   // samples inside the tile, in copy order
+    std::vector<long> nb;    // EXTENDED: neighbouring samples (own of parent, then its nb)
+    Box<T> bounds;
+    size_t chosenCap = 0;
+    std::vector<int> nbLeaves;  // GROUPED: neighbouring leaf positions (preorder)
+};
+
+template <typename T>
+struct Node {
+    int tile = -1;              // -1: a LOCAL inner node (no data)
+    std::vector<int> children;
+};
+
+template <typename T>
+class Tree {
+public:
+    TreeKind kind;
+    const T* P;
+    int dim;
+    size_t cap;
+    T r1;
+    std::vector<Tile<T>> tiles;
+    std::vector<Node<T>> nodes;   // node 0 is the root
+    std::vector<int> leafNodes;   // preorder
+    int maxDepth = 64;
+
+    Tree(TreeKind k, const T* p, long n, int d, size_t capacity, T radius,
+         const std::vector<size_t>& initTiles)
+        : kind(k), P(p), dim(d), cap(capacity), r1(radius) {
+        nodes.emplace_back();
+        std::vector<long> all((size_t)n);
+        std::iota(all.begin(), all.end(), 0L);
+        Box<T> g;
+        g.tight(P, all.data(), n, dim);
+        std::vector<int> created;
+        if (kind == TREE_LOCAL)
+            created = createLocal(all, initTiles, g);
+        else
+            created = createGrouped({&all, nullptr}, initTiles, g);
+        for (int t : created) addNode(0, t, 1);
+        collect(0);
+        if (kind == TREE_GROUPED) {
+            for (int ln : leafNodes)
+                for (int c : nodes[0].children) findNeighbourTiles(c, ln);
+        }
+    }
+
+    int leafPos(int node) const {
+        for (size_t i = 0; i < leafNodes.size(); ++i)
+            if (leafNodes[i] == node) return (int)i;
+        return -1;
+    }
+
+    // Samples a leaf evolves over: own, then neighbours (tile.hpp getAllSamples).
+    std::vector<long> evolveSamples(int leaf) const {
+        const Tile<T>& t = tiles[nodes[leafNodes[leaf]].tile];
+        std::vector<long> v = t.own;
+        if (kind == TREE_EXTENDED) v.insert(v.end(), t.nb.begin(), t.nb.end());
+        if (kind == TREE_GROUPED)
+            for (int nl : t.nbLeaves) {
+                const Tile<T>& o = tiles[nodes[leafNodes[nl]].tile];
+                v.insert(v.end(), o.own.begin(), o.own.end());
+            }
+        return v;
+    }
+
+private:
+    std::vector<int> createLocal(const std::vector<long>& pts, const std::vector<size_t>& nb,
+                                 const Box<T>& bb) {
+        size_t nT = 1;
+        for (size_t v : nb) nT *= v;
+        std::vector<std::vector<long>> bins(nT);
+        for (long i : pts) bins[binOf(P + i * (long)dim, nb, bb)].push_back(i);
+        std::vector<int> out;
+        for (size_t b = 0; b < nT; ++b) {
+            if (bins[b].size() < 2) continue;   // drop 0/1-point tiles (local_tile_tree.inl:153-168)
+            Tile<T> t;
+            t.own.swap(bins[b]);
+            t.bounds.tight(P, t.own.data(), (long)t.own.size(), dim);
+            if (t.own.size() < cap) t.chosenCap = t.bounds.countSpheresInside(r1);
+            tiles.push_back(std::move(t));
+            out.push_back((int)tiles.size() - 1);
+        }
+        return out;
+    }
+
+    // allocTiles + partitionSamples + reduceTilesAndAllocChosenSamples
+    // (grouped_tile_tree.inl:117-242, 444-510, 550-590)
+    std::vector<int> createGrouped(std::pair<const std::vector<long>*, const std::vector<long>*> src,
+                                   const std::vector<size_t>& nb, const Box<T>& bb) {
+        size_t nT = 1;
+        for (size_t v : nb) nT *= v;
+        std::vector<T> step(dim);
+        for (int k = 0; k < dim; ++k) step[k] = bb.dist[k] / static_cast<T>(nb[k]);
+        std::vector<Tile<T>> cells(nT);
+        std::vector<int> coord(dim, 0);
+        for (size_t i = 0; i < nT; ++i) {
+            if (i > 0) {
+                for (int d = 0; d < dim; ++d) {
+                    coord[d]++;
+                    if (coord[d] % (int)nb[d] == 0) coord[d] = 0; else break;
+                }
+            }
+            Box<T>& b = cells[i].bounds;
+            b.dim = dim;
+            b.mn.resize(dim);
+            b.mx.resize(dim);
+            for (int d = 0; d < dim; ++d) {
+                b.mn[d] = bb.mn[d] + coord[d] * step[d];
+                b.mx[d] = bb.mn[d] + (coord[d] + 1) * step[d];
+                if (coord[d] == (int)(nb[d] - 1)) b.mx[d] = bb.mx[d];
+            }
+            b.calcDistances();
+        }
+        const std::vector<long>* lists[2] = {src.first, src.second};
+        for (int k = 0; k < 2; ++k) {
+            if (!lists[k]) continue;
+            if (kind == TREE_GROUPED && k > 0) break;
+            for (long i : *lists[k]) {
+                const T* p = P + i * (long)dim;
+                if (kind == TREE_EXTENDED)
+                    for (auto& c : cells)
+                        if (c.bounds.isNearby(p, r1)) c.nb.push_back(i);
+                if (k == 0)
+                    for (auto& c : cells)
+                        if (c.bounds.isInside(p)) { c.own.push_back(i); break; }
+            }
+        }
+        std::vector<int> out;
+        for (auto& c : cells) {
+            if (c.own.size() < 2) continue;
+            if (c.own.size() < cap) c.chosenCap = c.bounds.countSpheresInside(r1);
+            tiles.push_back(std::move(c));
+            out.push_back((int)tiles.size() - 1);
+        }
+        return out;
+    }
+
+    void addNode(int parent, int tile, int level) {
+        if (level > maxDepth) throw std::runtime_error("tile tree exceeds maximum depth");
+        Node<T> nd;
+        int self = (int)nodes.size();
+        if (tiles[tile].own.size() < cap) {
+            nd.tile = tile;
+            nodes.push_back(nd);
+            nodes[parent].children.push_back(self);
+            return;
+        }
+        nd.tile = (kind == TREE_LOCAL) ? -1 : tile;
+        nodes.push_back(nd);
+        nodes[parent].children.push_back(self);
+        std::vector<size_t> nb(dim, 1);
+        nb[level % dim] = 2;
+        std::vector<int> subs;
+        if (kind == TREE_LOCAL) {
+            std::vector<long> own = tiles[tile].own;
+            Box<T> bb = tiles[tile].bounds;
+            subs = createLocal(own, nb, bb);
+        } else {
+            std::vector<long> own = tiles[tile].own;
+            std::vector<long> nbs = tiles[tile].nb;
+            Box<T> bb = tiles[tile].bounds;
+            subs = createGrouped({&own, kind == TREE_EXTENDED ? &nbs : nullptr}, nb, bb);
+        }
+        for (int s : subs) addNode(self, s, level + 1);
+    }
+
+    void collect(int node) {
+        if (node != 0 && nodes[node].children.empty()) {
+            if (nodes[node].tile < 0 || (kind != TREE_LOCAL && tiles[nodes[node].tile].own.size() >= cap))
+                throw std::runtime_error("subdivided tile lost all its samples");
+            leafNodes.push_back(node);
+        }
+        for (int c : nodes[node].children) collect(c);
+    }
+
+    // grouped_tile_tree.inl:592-624
+    void findNeighbourTiles(int curr, int ref) {
+        if (nodes[curr].tile == nodes[ref].tile) return;
+        const Tile<T>& ct = tiles[nodes[curr].tile];
+        Tile<T>& rt = tiles[nodes[ref].tile];
+        if (nodes[curr].children.empty()) {
+            if (rt.bounds.overlap(ct.bounds)) rt.nbLeaves.push_back(leafPos(curr));
+        } else if (ct.bounds.overlap(rt.bounds)) {
+            for (int c : nodes[curr].children) findNeighbourTiles(c, ref);
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// a14: tiled drivers.  rd/cpu/tiled/rd_tiled.hpp:756-1732; enum :102-110.
+// ---------------------------------------------------------------------------------
+enum TiledAlg {
+    TILED_LOCAL_TREE_MIXED_RD = 0,
+    TILED_GROUPED_TREE_MIXED_RD = 1,
+    TILED_LOCAL_TREE_LOCAL_RD = 2,
+    TILED_LOCAL_LIST = 3,
+    TILED_GROUPED_TREE_LOCAL_RD = 4,
+    EXT_TILE_TREE_MIXED_RD = 5,
+    EXT_TILE_TREE_LOCAL_RD = 6,   // not in the CPU reference: local RD over extended tiles
+};
+
+template <typename T>
+static long tiledRD(int alg, const T* P, long n, int dim, T r1, T r2, size_t cap,
+                    const std::vector<size_t>& tpd, bool refinement, long maxEvolveIter,
+                    std::vector<T>& out) {
+    out.clear();
+    if (alg == TILED_LOCAL_LIST) return 0;   // rd_tiled.hpp:1580-1590 (stub)
+    TreeKind kind = TREE_LOCAL;
+    bool mixed = false;
+    switch (alg) {
+        case TILED_LOCAL_TREE_MIXED_RD: kind = TREE_LOCAL; mixed = true; break;
+        case TILED_GROUPED_TREE_MIXED_RD: kind = TREE_GROUPED; mixed = true; break;
+        case TILED_LOCAL_TREE_LOCAL_RD: kind = TREE_LOCAL; mixed = false; break;
+        case TILED_GROUPED_TREE_LOCAL_RD: kind = TREE_GROUPED; mixed = false; break;
+        case EXT_TILE_TREE_MIXED_RD: kind = TREE_EXTENDED; mixed = true; break;
+        case EXT_TILE_TREE_LOCAL_RD: kind = TREE_EXTENDED; mixed = false; break;
+        default: throw std::runtime_error("unknown tiled algorithm");
+    }
+    Tree<T> tree(kind, P, n, dim, cap, r1, tpd);
+    const size_t L = tree.leafNodes.size();
+    std::vector<std::vector<long>> own(L), evs(L);
+    for (size_t l = 0; l < L; ++l) {
+        own[l] = tree.tiles[tree.nodes[tree.leafNodes[l]].tile].own;
+        evs[l] = tree.evolveSamples((int)l);
+    }
+    std::vector<std::vector<T>> S(L);
+    std::vector<long> K(L, 0);
+    long ns = 0;
+    if (mixed) {
+        for (size_t l = 0; l < L; ++l) {
+            K[l] = choose(P, own[l].data(), (long)own[l].size(), dim, r1, S[l], nullptr);
+            ns += K[l];
+        }
+        long oldCount = 0;
+        while (oldCount != ns) {
+            oldCount = ns;
+            for (size_t l = 0; l < L; ++l) {
+                std::vector<std::pair<const long*, long>> blocks{{evs[l].data(), (long)evs[l].size()}};
+                evolve(P, blocks, S[l].data(), K[l], dim, r1, maxEvolveIter);
+            }
+            // global decimateNoCopy over all leaves' lists in leaf order
+            std::vector<const T*> pts;
+            std::vector<size_t> owner;
+            for (size_t l = 0; l < L; ++l)
+                for (long k = 0; k < K[l]; ++k) { pts.push_back(S[l].data() + k * dim); owner.push_back(l); }
+            std::vector<unsigned char> alive(pts.size(), 1);
+            ns = decimateImpl(pts, alive, ns, dim, r2, true);
+            size_t g = 0;
+            for (size_t l = 0; l < L; ++l) {
+                std::vector<T> keep;
+                long nk = 0;
+                for (long k = 0; k < K[l]; ++k, ++g)
+                    if (alive[g]) { keep.insert(keep.end(), S[l].begin() + k * dim, S[l].begin() + (k + 1) * dim); ++nk; }
+                S[l].swap(keep);
+                K[l] = nk;
+            }
+        }
+    } else {
+        for (size_t l = 0; l < L; ++l) {
+            std::vector<std::pair<const long*, long>> blocks{{own[l].data(), (long)own[l].size()}};
+            if (kind != TREE_LOCAL && evs[l].size() > own[l].size())
+                blocks.push_back({evs[l].data() + own[l].size(), (long)(evs[l].size() - own[l].size())});
+            K[l] = ridgeDetection(P, blocks, dim, r1, r2, maxEvolveIter, S[l], nullptr);
+            ns += K[l];
+        }
+    }
+    for (size_t l = 0; l < L; ++l) out.insert(out.end(), S[l].begin(), S[l].begin() + K[l] * dim);
+    if (refinement) {   // rd_tiled.hpp:919-948: global decimate, then evolve over all points
+        std::vector<const T*> pts;
+        for (long k = 0; k < ns; ++k) pts.push_back(out.data() + k * dim);
+        std::vector<unsigned char> alive(pts.size(), 1);
+        long nk = decimateImpl(pts, alive, ns, dim, r2, false);
+        std::vector<T> keep;
+        for (long k = 0; k < ns; ++k)
+            if (alive[k]) keep.insert(keep.end(), out.begin() + k * dim, out.begin() + (k + 1) * dim);
+        out.swap(keep);
+        ns = nk;
+        std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
+        evolve(P, blocks, out.data(), ns, dim, r1, maxEvolveIter);
+    }
+    return ns;
+}
+
+}  // namespace oracle
+
+// =================================================================================
+// C ABI (ctypes).  Every function returns >= 0 on success, -1 on error; the message is
+// available from oracle_last_error().
+// =================================================================================
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+long guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+template <typename T>
+long chooseC(const T* P, long n, int dim, T r, T* S, long* idx) {
+    return guarded([&]() -> long {
+        if (!P || n < 1 || dim < 1) throw std::runtime_error("choose: invalid arguments");
+        std::vector<T> s;
+        std::vector<long> ci;
+        long K = oracle::choose<T>(P, nullptr, n, dim, r, s, &ci);
+        if (S) std::memcpy(S, s.data(), sizeof(T) * s.size());
+        if (idx) std::memcpy(idx, ci.data(), sizeof(long) * ci.size());
+        return K;
+    });
+}
+
+template <typename T>
+long evolveIterC(const T* P, long n, T* S, long K, int dim, T r, int* labels, int* counts, T* sums,
+                 unsigned char* moved) {
+    return guarded([&]() -> long {
+        std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
+        return oracle::evolveIteration<T>(P, blocks, S, K, dim, r, sums, counts, labels, moved);
+    });
+}
+
+template <typename T>
+long evolveC(const T* P, long n, T* S, long K, int dim, T r, long maxIter) {
+    return guarded([&]() -> long {
+        std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
+        return oracle::evolve<T>(P, blocks, S, K, dim, r, maxIter);
+    });
+}
+
+template <typename T>
+long evolveOmpIterC(const T* P, long n, T* S, long K, int dim, T r, int* counts, T* sums) {
+    return guarded([&]() -> long {
+        std::fill(sums, sums + K * dim, T(0));
+        std::fill(counts, counts + K, 0);
+        oracle::closestSphereCenterOmp<T>(P, n, S, K, dim, r * r, sums, counts);
+        return oracle::shiftTowardMassCenter<T>(S, sums, counts, K, dim, nullptr);
+    });
+}
+
+template <typename T>
+long decimateC(T* S, long K, int dim, T r2, int noCopyRule, long* kept) {
+    return guarded([&]() -> long {
+        std::vector<const T*> pts((size_t)K);
+        for (long k = 0; k < K; ++k) pts[k] = S + k * dim;
+        std::vector<unsigned char> alive((size_t)K, 1);
+        long nk = oracle::decimateImpl<T>(pts, alive, K, dim, r2, noCopyRule != 0);
+        std::vector<T> keep;
+        long j = 0;
+        for (long k = 0; k < K; ++k)
+            if (alive[k]) {
+                keep.insert(keep.end(), S + k * dim, S + (k + 1) * dim);
+                if (kept) kept[j] = k;
+                ++j;
+            }
+        std::memcpy(S, keep.data(), sizeof(T) * keep.size());
+        return nk;
+    });
+}
+
+template <typename T>
+long rdC(const T* P, long n, int dim, T r1, T r2, long maxEvolveIter, T* S, long* rounds) {
+    return guarded([&]() -> long {
+        if (!P || n < 1 || dim < 1) throw std::runtime_error("ridge detection: invalid arguments");
+        std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
+        std::vector<T> s;
+        long K = oracle::ridgeDetection<T>(P, blocks, dim, r1, r2, maxEvolveIter, s, rounds);
+        std::memcpy(S, s.data(), sizeof(T) * s.size());
+        return K;
+    });
+}
+
+template <typename T>
+long tiledC(int alg, const T* P, long n, int dim, T r1, T r2, long cap, const long* tpd, int refinement,
+            long maxEvolveIter, T* S, long Scap) {
+    return guarded([&]() -> long {
+        std::vector<size_t> t(tpd, tpd + dim);
+        std::vector<T> out;
+        long ns = oracle::tiledRD<T>(alg, P, n, dim, r1, r2, (size_t)cap, t, refinement != 0, maxEvolveIter, out);
+        if (ns > Scap) throw std::runtime_error("tiled: output capacity too small");
+        std::memcpy(S, out.data(), sizeof(T) * out.size());
+        return ns;
+    });
+}
+
+// Tree introspection: the whole tree is flattened into caller-provided arrays.
+//   leafInfo[4*l .. 4*l+3] = {ownCount, neighbourCount (EXTENDED: points, GROUPED: leaves),
+//                             chosenCap, 0}
+//   ownIdx   : concatenation of every leaf's own samples (input indices)
+//   nbIdx    : EXTENDED: concatenation of neighbour samples; GROUPED: neighbour leaf ids
+//   bounds   : 2*dim values per leaf, [min_0, max_0, min_1, max_1, ...]
+template <typename T>
+long treeC(int kind, const T* P, long n, int dim, T r1, long cap, const long* tpd, long* leafInfo,
+           long infoCap, long* ownIdx, long ownCap, long* nbIdx, long nbCap, T* bounds) {
+    return guarded([&]() -> long {
+        std::vector<size_t> t(tpd, tpd + dim);
+        oracle::Tree<T> tree((oracle::TreeKind)kind, P, n, dim, (size_t)cap, r1, t);
+        long L = (long)tree.leafNodes.size();
+        if (!leafInfo) return L;
+        if (L > infoCap) throw std::runtime_error("tree: leaf capacity too small");
+        long o = 0, b = 0;
+        for (long l = 0; l < L; ++l) {
+            const auto& tile = tree.tiles[tree.nodes[tree.leafNodes[l]].tile];
+            long nnb = kind == oracle::TREE_GROUPED ? (long)tile.nbLeaves.size() : (long)tile.nb.size();
+            leafInfo[4 * l] = (long)tile.own.size();
+            leafInfo[4 * l + 1] = nnb;
+            leafInfo[4 * l + 2] = (long)tile.chosenCap;
+            leafInfo[4 * l + 3] = 0;
+            if (o + (long)tile.own.size() > ownCap || b + nnb > nbCap)
+                throw std::runtime_error("tree: index capacity too small");
+            for (long i : tile.own) ownIdx[o++] = i;
+            if (kind == oracle::TREE_GROUPED)
+                for (int i : tile.nbLeaves) nbIdx[b++] = i;
+            else
+                for (long i : tile.nb) nbIdx[b++] = i;
+            for (int d = 0; d < dim; ++d) {
+                bounds[(2 * l) * dim + 2 * d] = tile.bounds.mn[d];
+                bounds[(2 * l) * dim + 2 * d + 1] = tile.bounds.mx[d];
+            }
+        }
+        return L;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+int oracle_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+#define ORACLE_DEFINE(SUF, T)                                                                        \
+    long oracle_choose_##SUF(const T* P, long n, int dim, T r, T* S, long* idx) {                    \
+        return chooseC<T>(P, n, dim, r, S, idx);                                                     \
+    }                                                                                                \
+    long oracle_evolve_iter_##SUF(const T* P, long n, T* S, long K, int dim, T r, int* labels,      \
+                                  int* counts, T* sums, unsigned char* moved) {                      \
+        return evolveIterC<T>(P, n, S, K, dim, r, labels, counts, sums, moved);                      \
+    }                                                                                                \
+    long oracle_evolve_##SUF(const T* P, long n, T* S, long K, int dim, T r, long maxIter) {         \
+        return evolveC<T>(P, n, S, K, dim, r, maxIter);                                              \
+    }                                                                                                \
+    long oracle_evolve_omp_iter_##SUF(const T* P, long n, T* S, long K, int dim, T r, int* counts,  \
+                                      T* sums) {                                                     \
+        return evolveOmpIterC<T>(P, n, S, K, dim, r, counts, sums);                                  \
+    }                                                                                                \
+    long oracle_decimate_##SUF(T* S, long K, int dim, T r2, int noCopyRule, long* kept) {            \
+        return decimateC<T>(S, K, dim, r2, noCopyRule, kept);                                        \
+    }                                                                                                \
+    long oracle_rd_##SUF(const T* P, long n, int dim, T r1, T r2, long maxEvolveIter, T* S,          \
+                         long* rounds) {                                                             \
+        return rdC<T>(P, n, dim, r1, r2, maxEvolveIter, S, rounds);                                  \
+    }                                                                                                \
+    long oracle_tiled_##SUF(int alg, const T* P, long n, int dim, T r1, T r2, long cap,              \
+                            const long* tpd, int refinement, long maxEvolveIter, T* S, long Scap) {  \
+        return tiledC<T>(alg, P, n, dim, r1, r2, cap, tpd, refinement, maxEvolveIter, S, Scap);      \
+    }                                                                                                \
+    long oracle_tree_##SUF(int kind, const T* P, long n, int dim, T r1, long cap, const long* tpd,   \
+                           long* leafInfo, long infoCap, long* ownIdx, long ownCap, long* nbIdx,     \
+                           long nbCap, T* bounds) {                                                  \
+        return treeC<T>(kind, P, n, dim, r1, cap, tpd, leafInfo, infoCap, ownIdx, ownCap, nbIdx,     \
+                        nbCap, bounds);                                                              \
+    }
+
+ORACLE_DEFINE(f32, float)
+ORACLE_DEFINE(f64, double)
+
+}  // extern "C"
