@@ -1,0 +1,142 @@
+/* rdhip.h -- C ABI of the MI355X (gfx950) ridge-detection library (librdhip.so).
+ *
+ * The entry points replace the reference's GPU classes and device-level facades of
+ * aosewski/RidgeDetection (paths relative to ridge_detection/):
+ *   rdhip_create/_set_points/_ridge_detection/_get_count/_get_points
+ *       <- rd::gpu::bruteForce::RidgeDetection<T,DIM,IN,OUT>
+ *          (rd/gpu/device/brute_force/simulation.cuh:35-112, simulation.inl:40-384)
+ *   rdhip_tiled_ridge_detection/_get_timers
+ *       <- rd::gpu::tiled::RidgeDetection<DIM,IN,OUT,ALG,POLICY,TILE,T>
+ *          (rd/gpu/device/tiled/simulation.cuh:33-241) and the CPU
+ *          rd::tiled::tiledRidgeDetection<ALG,T> (rd/cpu/tiled/rd_tiled.hpp:1763-1798)
+ *   rdhip_choose   <- DeviceChoose::choose     (rd/gpu/device/device_choose.cuh:55-82)
+ *   rdhip_evolve   <- DeviceEvolve::evolve     (rd/gpu/device/device_evolve.cuh:29-97)
+ *   rdhip_evolve_iteration
+ *                  <- DeviceClosestSphereKernel + DeviceShiftSphereKernel, one pass
+ *                     (rd/gpu/device/dispatch/dispatch_evolve.cuh:42-85)
+ *   rdhip_decimate <- DeviceDecimate::decimate (rd/gpu/device/device_decimate.cuh:50-117) and
+ *                     tiled::DeviceDecimate::globalDecimate (decimateNoCopy semantics)
+ *
+ * Conventions
+ *   - Every function returns RDHIP_OK (0) or a negative error code; the message of the
+ *     last failure on the calling thread is returned by rdhip_last_error().  Nothing
+ *     aborts the process (the reference exits through checkCudaErrors).
+ *   - A context owns all device memory it allocates and runs on the HIP stream given at
+ *     creation (NULL = a stream it creates).  A context is not thread-safe; distinct
+ *     contexts are independent (no process-global device state).
+ *   - Point layouts follow rd::DataMemoryLayout (rd/utils/memory.h:22-27):
+ *     ROW_MAJOR: point i at p[i*stride + d] (stride >= dim, 0 = dim);
+ *     COL_MAJOR / SOA: point i at p[d*stride + i] (stride >= n, 0 = n).
+ *   - Radii: r1 is the choose/evolve radius, r2 the decimate radius (reference names).
+ *   - max_evolve_iters = 0 runs every evolve to convergence, as the reference does.
+ */
+#ifndef RDHIP_H
+#define RDHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RDHIP_OK 0
+#define RDHIP_ERR_INVALID (-1)   /* bad argument */
+#define RDHIP_ERR_HIP (-2)       /* HIP runtime error (allocation, launch, ...) */
+#define RDHIP_ERR_RUNTIME (-3)   /* algorithmic limit (capacity, tree depth, ...) */
+#define RDHIP_ERR_STATE (-4)     /* call out of order (e.g. evolve before choose) */
+
+enum rdhip_dtype { RDHIP_F32 = 0, RDHIP_F64 = 1 };
+enum rdhip_layout { RDHIP_ROW_MAJOR = 0, RDHIP_COL_MAJOR = 1, RDHIP_SOA = 2 };
+
+/* Tiled algorithms: values of rd::tiled::TiledRDAlgorithm (rd/cpu/tiled/rd_tiled.hpp:102-110)
+ * plus local RD over extended tiles (the reference GPU's RD_LOCAL x RD_EXTENDED_TILE,
+ * rd/gpu/agent/agent_tiled_ridge_detection.cuh:44-88). */
+enum rdhip_tiled_algorithm {
+    RDHIP_TILED_LOCAL_TREE_MIXED_RD = 0,
+    RDHIP_TILED_GROUPED_TREE_MIXED_RD = 1,
+    RDHIP_TILED_LOCAL_TREE_LOCAL_RD = 2,
+    RDHIP_TILED_LOCAL_LIST = 3,
+    RDHIP_TILED_GROUPED_TREE_LOCAL_RD = 4,
+    RDHIP_EXT_TILE_TREE_MIXED_RD = 5,
+    RDHIP_EXT_TILE_TREE_LOCAL_RD = 6
+};
+
+/* Tile trees: rd::tiled::TiledLocalTree, TiledGroupTree<FIND_NEIGHBOURS>,
+ * TiledGroupTree<EXTENDED_TILE> (rd/cpu/tiled/{local,grouped}_tile_tree.hpp). */
+enum rdhip_tree_kind { RDHIP_TREE_LOCAL = 0, RDHIP_TREE_GROUPED = 1, RDHIP_TREE_EXTENDED = 2 };
+
+/* Decimation rules: rd::decimate and rd::decimateNoCopy (rd/cpu/brute_force/decimate.hpp). */
+enum rdhip_decimate_rule { RDHIP_DECIMATE = 0, RDHIP_DECIMATE_NO_COPY = 1 };
+
+typedef struct rdhip_ctx rdhip_ctx;
+
+/* Counterpart of TiledRidgeDetectionTimers (rd/gpu/agent/agent_tiled_ridge_detection.cuh:96-102)
+ * and of the (whole, build, rdTiles, refinement) tuple of rd_tiled.hpp:1763-1798. */
+typedef struct rdhip_timers {
+    double whole_ms;
+    double build_ms;
+    double choose_ms;
+    double evolve_ms;
+    double decimate_ms;
+    double refine_ms;
+    int64_t evolve_iterations;
+    int64_t rounds;
+} rdhip_timers;
+
+int rdhip_version(void);
+const char* rdhip_last_error(void);
+
+/* Context: value type (RDHIP_F32/F64), point dimension in [1, 12], HIP stream (hipStream_t
+ * or NULL). */
+int rdhip_create(rdhip_ctx** out, int dtype, int dim, void* hip_stream);
+int rdhip_destroy(rdhip_ctx* ctx);
+
+/* Input cloud.  on_device != 0: `points` is a device pointer (copied, not retained). */
+int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, int64_t stride,
+                     int on_device);
+
+/* Brute-force ridge detection: choose, then rounds of evolve + decimate until the number of
+ * chosen points is stable (rd/cpu/brute_force/ridge_detection.inl:67-146). */
+int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evolve_iters,
+                          int64_t* out_count);
+
+/* Tiled ridge detection (tree build, per-tile choose/evolve, local or global decimation,
+ * optional end-phase refinement: decimate + evolve over all points). tiles_per_dim has dim
+ * entries; max_tile_capacity: a tile with at least this many own points is split. */
+int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double r2,
+                                int64_t max_tile_capacity, const int64_t* tiles_per_dim,
+                                int refinement, int64_t max_evolve_iters, int64_t* out_count);
+
+/* Results of the last run (or phase). */
+int rdhip_get_count(rdhip_ctx* ctx, int64_t* out_count);
+int rdhip_get_points(rdhip_ctx* ctx, void* out, int layout, int64_t stride, int on_device);
+int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out);
+
+/* ---- phase-level entry points on the whole cloud (parity tests, custom drivers) ---- */
+
+/* Greedy choose with radius r; out_indices (host, optional, capacity n) receives the
+ * chosen points' input indices in ascending order. */
+int rdhip_choose(rdhip_ctx* ctx, double r, int64_t* out_count, int32_t* out_indices);
+
+/* Replaces the chosen set (host pointer, `count` points in `layout`). */
+int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout);
+
+/* Evolve the chosen set with radius r until convergence (or max_iters > 0 iterations). */
+int rdhip_evolve(rdhip_ctx* ctx, double r, int64_t max_iters, int64_t* out_iters);
+
+/* One evolve iteration.  Optional host outputs: labels[n] (chosen index of every point, or
+ * -1 outside every r-ball), counts[K], sums[K*dim] (ROW_MAJOR, value type of the context),
+ * moved[K] (bit d set when coordinate d moved).  out_flag receives the reference's
+ * contFlag. */
+int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* counts, void* sums,
+                           uint16_t* moved, int* out_flag);
+
+/* Decimate the chosen set with radius r2 and `rule`; keep (host, optional, capacity K)
+ * receives 1 for kept and 0 for removed points, in the pre-decimation order. */
+int rdhip_decimate(rdhip_ctx* ctx, double r2, int rule, uint8_t* keep, int64_t* out_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RDHIP_H */

@@ -1,0 +1,197 @@
+"""ctypes binding of librdhip.so (include/rdhip.h).
+
+This is plumbing for the tests and the benchmark: the product is the C ABI.  Loading
+fails loudly when the library has not been built -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librdhip.so")
+
+RDHIP_F32, RDHIP_F64 = 0, 1
+ROW_MAJOR, COL_MAJOR, SOA = 0, 1, 2
+DECIMATE, DECIMATE_NO_COPY = 0, 1
+TILED_LOCAL_TREE_MIXED_RD = 0
+TILED_GROUPED_TREE_MIXED_RD = 1
+TILED_LOCAL_TREE_LOCAL_RD = 2
+TILED_LOCAL_LIST = 3
+TILED_GROUPED_TREE_LOCAL_RD = 4
+EXT_TILE_TREE_MIXED_RD = 5
+EXT_TILE_TREE_LOCAL_RD = 6
+
+EXPORTS = [
+    "rdhip_version", "rdhip_last_error", "rdhip_create", "rdhip_destroy", "rdhip_set_points",
+    "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
+    "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
+    "rdhip_decimate",
+]
+
+
+class Timers(C.Structure):
+    _fields_ = [("whole_ms", C.c_double), ("build_ms", C.c_double), ("choose_ms", C.c_double),
+                ("evolve_ms", C.c_double), ("decimate_ms", C.c_double), ("refine_ms", C.c_double),
+                ("evolve_iterations", C.c_int64), ("rounds", C.c_int64)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m ridgedetection_amd.build`")
+    lib = C.CDLL(path)
+    vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
+    sig = {
+        "rdhip_version": (i32, []),
+        "rdhip_last_error": (C.c_char_p, []),
+        "rdhip_create": (i32, [C.POINTER(vp), i32, i32, vp]),
+        "rdhip_destroy": (i32, [vp]),
+        "rdhip_set_points": (i32, [vp, vp, i64, i32, i64, i32]),
+        "rdhip_ridge_detection": (i32, [vp, C.c_double, C.c_double, i64, C.POINTER(i64)]),
+        "rdhip_tiled_ridge_detection": (i32, [vp, i32, C.c_double, C.c_double, i64, vp, i32, i64,
+                                              C.POINTER(i64)]),
+        "rdhip_get_count": (i32, [vp, C.POINTER(i64)]),
+        "rdhip_get_points": (i32, [vp, vp, i32, i64, i32]),
+        "rdhip_get_timers": (i32, [vp, C.POINTER(Timers)]),
+        "rdhip_choose": (i32, [vp, C.c_double, C.POINTER(i64), vp]),
+        "rdhip_set_chosen": (i32, [vp, vp, i64, i32]),
+        "rdhip_evolve": (i32, [vp, C.c_double, i64, C.POINTER(i64)]),
+        "rdhip_evolve_iteration": (i32, [vp, C.c_double, vp, vp, vp, vp, C.POINTER(i32)]),
+        "rdhip_decimate": (i32, [vp, C.c_double, i32, vp, C.POINTER(i64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class RDError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One rdhip context (a cloud on the GPU).  dtype: np.float32 or np.float64."""
+
+    def __init__(self, dim: int, dtype=np.float32, stream: int | None = None):
+        self.lib = load()
+        self.dim = dim
+        self.dtype = np.dtype(dtype)
+        h = C.c_void_p()
+        code = RDHIP_F32 if self.dtype == np.float32 else RDHIP_F64
+        self._check(self.lib.rdhip_create(C.byref(h), code, dim, C.c_void_p(stream) if stream else None))
+        self.h = h
+        self.n = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RDError(f"rdhip error {rc}: {self.lib.rdhip_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rdhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- input / output ------------------------------------------------------------
+    def set_points(self, P, layout=ROW_MAJOR, stride=0):
+        P = np.ascontiguousarray(P, dtype=self.dtype)
+        self.n = P.shape[0] if layout == ROW_MAJOR else P.shape[-1]
+        self._keep = P
+        self._check(self.lib.rdhip_set_points(self.h, _ptr(P), self.n, layout, stride, 0))
+
+    def set_points_device(self, ptr: int, n: int, layout=ROW_MAJOR, stride=0):
+        self.n = n
+        self._check(self.lib.rdhip_set_points(self.h, C.c_void_p(ptr), n, layout, stride, 1))
+
+    def count(self) -> int:
+        k = C.c_int64()
+        self._check(self.lib.rdhip_get_count(self.h, C.byref(k)))
+        return k.value
+
+    def points(self, layout=ROW_MAJOR) -> np.ndarray:
+        k = self.count()
+        out = np.empty((k, self.dim) if layout == ROW_MAJOR else (self.dim, k), dtype=self.dtype)
+        if k:
+            self._check(self.lib.rdhip_get_points(self.h, _ptr(out), layout, 0, 0))
+        return out
+
+    def timers(self) -> dict:
+        t = Timers()
+        self._check(self.lib.rdhip_get_timers(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in Timers._fields_}
+
+    # -- drivers -------------------------------------------------------------------
+    def ridge_detection(self, r1, r2, max_evolve_iters=0):
+        k = C.c_int64()
+        self._check(self.lib.rdhip_ridge_detection(self.h, float(r1), float(r2), int(max_evolve_iters),
+                                                   C.byref(k)))
+        return self.points()
+
+    def tiled(self, algorithm, r1, r2, max_tile_capacity, tiles_per_dim, refinement=False, max_evolve_iters=0):
+        tpd = np.ascontiguousarray(tiles_per_dim, dtype=np.int64)
+        k = C.c_int64()
+        self._check(self.lib.rdhip_tiled_ridge_detection(self.h, int(algorithm), float(r1), float(r2),
+                                                         int(max_tile_capacity), _ptr(tpd), int(bool(refinement)),
+                                                         int(max_evolve_iters), C.byref(k)))
+        return self.points()
+
+    # -- phases --------------------------------------------------------------------
+    def choose(self, r):
+        k = C.c_int64()
+        idx = np.empty(max(self.n, 1), dtype=np.int32)
+        self._check(self.lib.rdhip_choose(self.h, float(r), C.byref(k), _ptr(idx)))
+        return self.points(), idx[:k.value].copy()
+
+    def set_chosen(self, S):
+        S = np.ascontiguousarray(S, dtype=self.dtype)
+        self._check(self.lib.rdhip_set_chosen(self.h, _ptr(S), S.shape[0], ROW_MAJOR))
+
+    def evolve(self, r, max_iters=0):
+        it = C.c_int64()
+        self._check(self.lib.rdhip_evolve(self.h, float(r), int(max_iters), C.byref(it)))
+        return self.points(), it.value
+
+    def evolve_iteration(self, r):
+        K = self.count()
+        labels = np.empty(max(self.n, 1), dtype=np.int32)
+        counts = np.empty(max(K, 1), dtype=np.int32)
+        sums = np.empty((max(K, 1), self.dim), dtype=self.dtype)
+        moved = np.empty(max(K, 1), dtype=np.uint16)
+        flag = C.c_int()
+        self._check(self.lib.rdhip_evolve_iteration(self.h, float(r), _ptr(labels), _ptr(counts), _ptr(sums),
+                                                    _ptr(moved), C.byref(flag)))
+        return (self.points(), flag.value, counts[:K].copy(), sums[:K].copy(), labels[:self.n].copy(),
+                moved[:K].copy())
+
+    def decimate(self, r2, rule=DECIMATE):
+        K = self.count()
+        keep = np.empty(max(K, 1), dtype=np.uint8)
+        k = C.c_int64()
+        self._check(self.lib.rdhip_decimate(self.h, float(r2), int(rule), _ptr(keep), C.byref(k)))
+        return self.points(), keep[:K].astype(bool)

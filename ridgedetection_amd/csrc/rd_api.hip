@@ -1,0 +1,304 @@
+// rd_api.hip -- the C ABI (include/rdhip.h) over the device engine.
+#include "../../include/rdhip.h"
+#include "rd_engine.hpp"
+#include "rd_tiled.hpp"
+
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace rdhip;
+
+struct rdhip_ctx {
+    int dtype = RDHIP_F32;
+    int dim = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+    std::unique_ptr<Engine<float>> f32;
+    std::unique_ptr<Engine<double>> f64;
+    bool hasPoints = false;
+    bool hasChosen = false;
+    Timers timers;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(RDHIP_ERR_HIP, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(RDHIP_ERR_INVALID, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(RDHIP_ERR_RUNTIME, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(RDHIP_ERR_RUNTIME, e.what());
+    }
+}
+
+// Runs `body(engine)` on the context's engine of the right value type.
+template <typename F>
+int withEngine(rdhip_ctx* ctx, F&& body) {
+    if (!ctx) return fail(RDHIP_ERR_INVALID, "null context");
+    return guarded([&]() -> int {
+        if (ctx->dtype == RDHIP_F32) return body(*ctx->f32, (float)0);
+        return body(*ctx->f64, (double)0);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdhip_version(void) { return 100; }
+
+const char* rdhip_last_error(void) { return g_error.c_str(); }
+
+int rdhip_create(rdhip_ctx** out, int dtype, int dim, void* hip_stream) {
+    if (!out) return fail(RDHIP_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if (dtype != RDHIP_F32 && dtype != RDHIP_F64) return fail(RDHIP_ERR_INVALID, "dtype must be RDHIP_F32 or RDHIP_F64");
+    if (dim < 1 || dim > kMaxDim) return fail(RDHIP_ERR_INVALID, "dimension must be in [1, 12]");
+    return guarded([&]() -> int {
+        std::unique_ptr<rdhip_ctx> c(new rdhip_ctx());
+        c->dtype = dtype;
+        c->dim = dim;
+        if (hip_stream) {
+            c->stream = static_cast<hipStream_t>(hip_stream);
+        } else {
+            check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            c->ownStream = true;
+        }
+        if (dtype == RDHIP_F32)
+            c->f32.reset(new Engine<float>(dim, c->stream));
+        else
+            c->f64.reset(new Engine<double>(dim, c->stream));
+        *out = c.release();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_destroy(rdhip_ctx* ctx) {
+    if (!ctx) return RDHIP_OK;
+    return guarded([&]() -> int {
+        hipStream_t s = ctx->stream;
+        bool own = ctx->ownStream;
+        (void)hipStreamSynchronize(s);
+        ctx->f32.reset();
+        ctx->f64.reset();
+        if (own) (void)hipStreamDestroy(s);
+        delete ctx;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, int64_t stride, int on_device) {
+    if (ctx && n > 0 && !points) return fail(RDHIP_ERR_INVALID, "null points");
+    if (n < 0 || n >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "point count out of range");
+    if (layout < RDHIP_ROW_MAJOR || layout > RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "unknown layout");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (layout == RDHIP_ROW_MAJOR && stride != 0 && stride < ctx->dim)
+            return fail(RDHIP_ERR_INVALID, "row stride smaller than the dimension");
+        if (layout != RDHIP_ROW_MAJOR && stride != 0 && stride < n)
+            return fail(RDHIP_ERR_INVALID, "column stride smaller than the point count");
+        e.setPoints(static_cast<const T*>(points), n, layout, stride, on_device != 0);
+        ctx->hasPoints = true;
+        ctx->hasChosen = false;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evolve_iters, int64_t* out_count) {
+    if (!(r1 >= 0) || !(r2 >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
+    if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasPoints || e.numPoints() < 1) return fail(RDHIP_ERR_STATE, "no input points");
+        e.timers = Timers();
+        e.setSingleSegment((T)r1, (T)r2);
+        RunOptions opt = e.options;
+        opt.maxEvolveIters = max_evolve_iters;
+        e.ridgeDetectionLocal(opt);
+        ctx->hasChosen = true;
+        if (out_count) *out_count = e.totalCount();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double r2, int64_t max_tile_capacity,
+                                const int64_t* tiles_per_dim, int refinement, int64_t max_evolve_iters,
+                                int64_t* out_count) {
+    if (!(r1 > 0) || !(r2 >= 0)) return fail(RDHIP_ERR_INVALID, "r1 must be positive and r2 non-negative");
+    if (algorithm < RDHIP_TILED_LOCAL_TREE_MIXED_RD || algorithm > RDHIP_EXT_TILE_TREE_LOCAL_RD)
+        return fail(RDHIP_ERR_INVALID, "unknown tiled algorithm");
+    if (max_tile_capacity < 3) return fail(RDHIP_ERR_INVALID, "max_tile_capacity must be at least 3");
+    if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
+    if (ctx && !tiles_per_dim) return fail(RDHIP_ERR_INVALID, "null tiles_per_dim");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasPoints || e.numPoints() < 1) return fail(RDHIP_ERR_STATE, "no input points");
+        std::vector<int64_t> tpd(tiles_per_dim, tiles_per_dim + ctx->dim);
+        for (int64_t v : tpd)
+            if (v < 1 || v > 4096) return fail(RDHIP_ERR_INVALID, "tiles per dimension must be in [1, 4096]");
+        e.timers = Timers();
+        TiledParams<T> p;
+        p.algorithm = algorithm;
+        p.r1 = (T)r1;
+        p.r2 = (T)r2;
+        p.maxTileCapacity = max_tile_capacity;
+        p.tilesPerDim = tpd;
+        p.refinement = refinement != 0;
+        p.maxEvolveIters = max_evolve_iters;
+        tiled_ridge_detection<T>(e, p);
+        ctx->hasChosen = true;
+        if (out_count) *out_count = e.totalCount();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_get_count(rdhip_ctx* ctx, int64_t* out_count) {
+    if (!out_count) return fail(RDHIP_ERR_INVALID, "null output");
+    return withEngine(ctx, [&](auto& e, auto) -> int {
+        *out_count = ctx->hasChosen ? e.totalCount() : 0;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_get_points(rdhip_ctx* ctx, void* out, int layout, int64_t stride, int on_device) {
+    if (layout < RDHIP_ROW_MAJOR || layout > RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "unknown layout");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no result yet");
+        const int64_t K = e.totalCount();
+        if (K == 0) return RDHIP_OK;
+        if (!out) return fail(RDHIP_ERR_INVALID, "null output");
+        if (layout == RDHIP_ROW_MAJOR && stride != 0 && stride < ctx->dim)
+            return fail(RDHIP_ERR_INVALID, "row stride smaller than the dimension");
+        if (layout != RDHIP_ROW_MAJOR && stride != 0 && stride < K)
+            return fail(RDHIP_ERR_INVALID, "column stride smaller than the point count");
+        e.getChosen(static_cast<T*>(out), layout, on_device != 0, stride);
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
+    if (!out) return fail(RDHIP_ERR_INVALID, "null output");
+    return withEngine(ctx, [&](auto& e, auto) -> int {
+        out->whole_ms = e.timers.wholeMs;
+        out->build_ms = e.timers.buildMs;
+        out->choose_ms = e.timers.chooseMs;
+        out->evolve_ms = e.timers.evolveMs;
+        out->decimate_ms = e.timers.decimateMs;
+        out->refine_ms = e.timers.refineMs;
+        out->evolve_iterations = e.timers.evolveIterations;
+        out->rounds = e.timers.rounds;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_choose(rdhip_ctx* ctx, double r, int64_t* out_count, int32_t* out_indices) {
+    if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasPoints || e.numPoints() < 1) return fail(RDHIP_ERR_STATE, "no input points");
+        e.setSingleSegment((T)r, (T)r);
+        e.choose();
+        ctx->hasChosen = true;
+        const int64_t K = e.totalCount();
+        if (out_count) *out_count = K;
+        if (out_indices) {
+            std::vector<int> idx = e.chosenIndicesHost(0);
+            for (size_t i = 0; i < idx.size(); ++i) out_indices[i] = idx[i];
+        }
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout) {
+    if (count < 0) return fail(RDHIP_ERR_INVALID, "negative count");
+    if (count > 0 && !chosen) return fail(RDHIP_ERR_INVALID, "null chosen points");
+    if (layout != RDHIP_ROW_MAJOR && layout != RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "layout must be ROW_MAJOR or SOA");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasPoints) return fail(RDHIP_ERR_STATE, "no input points");
+        if (count > e.numPoints() + 4096) return fail(RDHIP_ERR_INVALID, "more chosen points than capacity");
+        if (e.numSegments() != 1) e.setSingleSegment(T(0), T(0));
+        e.setChosen(0, static_cast<const T*>(chosen), count, layout);
+        ctx->hasChosen = true;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_evolve(rdhip_ctx* ctx, double r, int64_t max_iters, int64_t* out_iters) {
+    if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    if (max_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points (call rdhip_choose first)");
+        e.setRadii({(T)r}, {(T)r});
+        long long it = e.evolve(max_iters);
+        if (out_iters) *out_iters = it;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* counts, void* sums, uint16_t* moved,
+                           int* out_flag) {
+    if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points (call rdhip_choose first)");
+        e.setRadii({(T)r}, {(T)r});
+        std::vector<int> lab, cnt;
+        std::vector<T> sm;
+        std::vector<unsigned short> mv;
+        int flag = e.evolveIteration(labels ? &lab : nullptr, counts ? &cnt : nullptr, sums ? &sm : nullptr,
+                                     moved ? &mv : nullptr);
+        const int64_t K = e.totalCount();
+        const int64_t n = e.numPoints();
+        if (labels)
+            for (int64_t i = 0; i < n; ++i) labels[i] = lab[i];
+        if (counts)
+            for (int64_t k = 0; k < K; ++k) counts[k] = cnt[k];
+        if (sums) {
+            T* out = static_cast<T*>(sums);
+            const int64_t stride = (int64_t)sm.size() / ctx->dim;
+            for (int64_t k = 0; k < K; ++k)
+                for (int d = 0; d < ctx->dim; ++d) out[k * ctx->dim + d] = sm[d * stride + k];
+        }
+        if (moved)
+            for (int64_t k = 0; k < K; ++k) moved[k] = mv[k];
+        if (out_flag) *out_flag = flag;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_decimate(rdhip_ctx* ctx, double r2, int rule, uint8_t* keep, int64_t* out_count) {
+    if (!(r2 >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    if (rule != RDHIP_DECIMATE && rule != RDHIP_DECIMATE_NO_COPY) return fail(RDHIP_ERR_INVALID, "unknown rule");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points");
+        const int64_t K0 = e.totalCount();
+        e.setRadii({(T)r2}, {(T)r2});
+        std::vector<unsigned char> kp;
+        e.decimate(rule == RDHIP_DECIMATE_NO_COPY, keep ? &kp : nullptr);
+        if (keep)
+            for (int64_t k = 0; k < K0; ++k) keep[k] = kp[k];
+        if (out_count) *out_count = e.totalCount();
+        return RDHIP_OK;
+    });
+}
+
+}  // extern "C"

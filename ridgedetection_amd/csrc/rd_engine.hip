@@ -1,0 +1,606 @@
+// rd_engine.hip -- host-side orchestration of the ridge-detection phases on one GPU.
+//
+// An Engine owns a SoA point buffer and a set of segments over it.  A segment is the
+// unit the reference runs its loop on: the whole cloud (brute force,
+// rd/gpu/device/brute_force/simulation.inl:183-384), one leaf tile (tiled RD,
+// rd/cpu/tiled/rd_tiled.hpp:756-1732) or one cloud of a batch.  Every phase runs on all
+// segments in one launch per kernel; all counts stay on the device and the host reads
+// them back once per round (or once every few evolve iterations to test convergence).
+#include "rd_engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace rdhip {
+
+namespace {
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// An uncapped evolve that has not converged after this many iterations is reported as an
+// error instead of looping forever.
+constexpr long long kEvolveGuard = 100000;
+
+__global__ void fill_int_kernel(int* __restrict__ a, int64_t n, int v, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = v;
+}
+
+// ROW_MAJOR (n x dim, row stride `stride`) -> SoA (dim x pStride)
+template <typename T>
+__global__ void row_to_soa_kernel(const T* __restrict__ src, int64_t n, int dim, int64_t stride,
+                                  T* __restrict__ dst, int64_t pStride) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        for (int d = 0; d < dim; ++d) dst[d * pStride + i] = src[i * stride + d];
+}
+
+// gathers the live chosen points of every segment into a dense output
+template <typename T>
+__global__ void gather_chosen_kernel(const T* __restrict__ S, int64_t sStride, const SegDesc* __restrict__ segs,
+                                     const int* __restrict__ segK, const int64_t* __restrict__ outOff, int dim,
+                                     T* __restrict__ out, int layout, int64_t outStride) {
+    const int s = blockIdx.x;
+    const SegDesc sd = segs[s];
+    const int K = segK[s];
+    const int64_t o = outOff[s];
+    for (int k = threadIdx.x; k < K; k += 256)
+        for (int d = 0; d < dim; ++d) {
+            T v = S[d * sStride + sd.sOff + k];
+            if (layout == LAYOUT_ROW_MAJOR)
+                out[(o + k) * outStride + d] = v;
+            else
+                out[d * outStride + o + k] = v;
+        }
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
+
+}  // namespace
+
+template <typename T>
+Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
+    if (dim < 1 || dim > kMaxDim) throw std::invalid_argument("dimension must be in [1, 12]");
+    flags_.reserve(4);
+    dN_.reserve(1);
+    dM_.reserve(1);
+}
+
+template <typename T>
+Engine<T>::~Engine() {
+    if (decWs_.nbr) (void)hipFree(decWs_.nbr);
+    for (auto* b : chooseItems_) delete b;
+}
+
+template <typename T>
+void Engine<T>::setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice) {
+    if (n < 0) throw std::invalid_argument("negative point count");
+    nPoints_ = n;
+    pStride_ = round_up(std::max<int64_t>(n, 1), 64);
+    P_.reserve((size_t)pStride_ * dim_);
+    if (n == 0) return;
+    if (layout == LAYOUT_ROW_MAJOR) {
+        if (stride <= 0) stride = dim_;
+        const T* dsrc = src;
+        DevBuf<T> staging;
+        if (!onDevice) {
+            staging.reserve((size_t)n * stride);
+            check(hipMemcpyAsync(staging.get(), src, sizeof(T) * n * stride, hipMemcpyHostToDevice, st_), "upload");
+            dsrc = staging.get();
+        }
+        hipLaunchKernelGGL((row_to_soa_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st_, dsrc, n, dim_, stride,
+                           P_.get(), pStride_);
+        check(hipGetLastError(), "row_to_soa");
+        check(hipStreamSynchronize(st_), "setPoints");
+    } else if (layout == LAYOUT_COL_MAJOR || layout == LAYOUT_SOA) {
+        if (stride <= 0) stride = n;
+        check(hipMemcpy2DAsync(P_.get(), sizeof(T) * pStride_, src, sizeof(T) * stride, sizeof(T) * n, dim_,
+                               onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_),
+              "upload");
+        check(hipStreamSynchronize(st_), "setPoints");
+    } else {
+        throw std::invalid_argument("unsupported memory layout");
+    }
+}
+
+template <typename T>
+void Engine<T>::setSingleSegment(T r1, T r2) {
+    SegDesc s{};
+    s.pOff = 0;
+    s.nP = (int32_t)nPoints_;
+    s.nOwn = (int32_t)nPoints_;
+    s.group = 0;
+    setSegments({s}, {std::max<int64_t>(nPoints_, 1)}, {r1}, {r2});
+}
+
+template <typename T>
+void Engine<T>::setSegments(const std::vector<SegDesc>& segs, const std::vector<int64_t>& capacities,
+                            const std::vector<T>& r1, const std::vector<T>& r2) {
+    const int ns = (int)segs.size();
+    if (capacities.size() != segs.size() || r1.size() != segs.size() || r2.size() != segs.size())
+        throw std::invalid_argument("segment arrays differ in length");
+    segs_ = segs;
+    r1_ = r1;
+    r2_ = r2;
+    int64_t off = 0;
+    for (int s = 0; s < ns; ++s) {
+        segs_[s].sOff = off;
+        int64_t cap = round_up(std::max<int64_t>(capacities[s], 1), kChosenTile);
+        if (cap > 0x7fffffffLL) throw std::invalid_argument("segment capacity too large");
+        segs_[s].sCap = (int32_t)cap;
+        off += cap;
+    }
+    totalSlots_ = off;
+    sStride_ = round_up(std::max<int64_t>(totalSlots_, 1), 64);
+    if (totalSlots_ >= (int64_t(1) << 30)) throw std::invalid_argument("too many chosen slots");
+    // groups: consecutive runs of equal `group`
+    groupBegin_.clear();
+    groupEnd_.clear();
+    for (int s = 0; s < ns; ++s) {
+        if (s == 0 || segs_[s].group != segs_[s - 1].group) {
+            if (s > 0) groupEnd_.push_back(s);
+            groupBegin_.push_back(s);
+        }
+        segs_[s].group = (int32_t)groupBegin_.size() - 1;
+    }
+    if (ns > 0) groupEnd_.push_back(ns);
+    const int ng = (int)groupBegin_.size();
+    std::vector<T> r1sq(ns), r2sq(ng);
+    for (int s = 0; s < ns; ++s) r1sq[s] = r1_[s] * r1_[s];
+    for (int g = 0; g < ng; ++g) r2sq[g] = r2_[groupBegin_[g]] * r2_[groupBegin_[g]];
+    std::vector<int> slotSeg((size_t)totalSlots_);
+    for (int s = 0; s < ns; ++s)
+        std::fill(slotSeg.begin() + segs_[s].sOff, slotSeg.begin() + segs_[s].sOff + segs_[s].sCap, s);
+
+    dSegs_.reserve(std::max(ns, 1));
+    dSegK_.reserve(std::max(ns, 1));
+    dSegDone_.reserve(std::max(ns, 1));
+    dSegIter_.reserve(std::max(ns, 1));
+    dSegMoved_.reserve(std::max(ns, 1));
+    dSegR1Sqr_.reserve(std::max(ns, 1));
+    dGroupR2Sqr_.reserve(std::max(ng, 1));
+    dGroupBegin_.reserve(std::max(ng, 1));
+    dGroupEnd_.reserve(std::max(ng, 1));
+    dSlotSeg_.reserve(std::max<int64_t>(totalSlots_, 1));
+    S_.reserve((size_t)sStride_ * dim_);
+    chosenIdx_.reserve((size_t)std::max<int64_t>(totalSlots_, 1));
+    if (ns > 0) {
+        check(hipMemcpyAsync(dSegs_.get(), segs_.data(), sizeof(SegDesc) * ns, hipMemcpyHostToDevice, st_), "segs");
+        check(hipMemcpyAsync(dSegR1Sqr_.get(), r1sq.data(), sizeof(T) * ns, hipMemcpyHostToDevice, st_), "r1");
+        check(hipMemcpyAsync(dGroupR2Sqr_.get(), r2sq.data(), sizeof(T) * ng, hipMemcpyHostToDevice, st_), "r2");
+        check(hipMemcpyAsync(dGroupBegin_.get(), groupBegin_.data(), sizeof(int) * ng, hipMemcpyHostToDevice, st_), "g");
+        check(hipMemcpyAsync(dGroupEnd_.get(), groupEnd_.data(), sizeof(int) * ng, hipMemcpyHostToDevice, st_), "g");
+        check(hipMemcpyAsync(dSlotSeg_.get(), slotSeg.data(), sizeof(int) * totalSlots_, hipMemcpyHostToDevice, st_),
+              "slots");
+        std::vector<int> zeros(ns, 0);
+        check(hipMemcpyAsync(dSegK_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "K");
+    }
+    buildWorkItems();
+    ensureEvolveScratch();
+    check(hipStreamSynchronize(st_), "setSegments");
+}
+
+template <typename T>
+void Engine<T>::setRadii(const std::vector<T>& r1, const std::vector<T>& r2) {
+    const int ns = (int)segs_.size();
+    if ((int)r1.size() != ns || (int)r2.size() != ns) throw std::invalid_argument("radius arrays differ in length");
+    r1_ = r1;
+    r2_ = r2;
+    const int ng = (int)groupBegin_.size();
+    std::vector<T> r1sq(ns), r2sq(ng);
+    for (int s = 0; s < ns; ++s) r1sq[s] = r1_[s] * r1_[s];
+    for (int g = 0; g < ng; ++g) r2sq[g] = r2_[groupBegin_[g]] * r2_[groupBegin_[g]];
+    if (ns > 0) {
+        check(hipMemcpyAsync(dSegR1Sqr_.get(), r1sq.data(), sizeof(T) * ns, hipMemcpyHostToDevice, st_), "r1");
+        check(hipMemcpyAsync(dGroupR2Sqr_.get(), r2sq.data(), sizeof(T) * ng, hipMemcpyHostToDevice, st_), "r2");
+        check(hipStreamSynchronize(st_), "setRadii");
+    }
+}
+
+template <typename T>
+void Engine<T>::buildWorkItems() {
+    std::vector<WorkItem> items;
+    const int per = kScanThreads * std::max(1, options.ppt);
+    for (int s = 0; s < (int)segs_.size(); ++s)
+        for (int st = 0; st < segs_[s].nP; st += per) items.push_back({s, st, std::min(segs_[s].nP, st + per), 0});
+    numItems_ = (int)items.size();
+    dItems_.reserve(std::max<size_t>(items.size(), 1));
+    if (!items.empty())
+        check(hipMemcpyAsync(dItems_.get(), items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, st_),
+              "items");
+    // choose filter items for the chunked (large segment) path
+    for (auto* b : chooseItems_) delete b;
+    chooseItems_.clear();
+}
+
+template <typename T>
+void Engine<T>::ensureEvolveScratch() {
+    int64_t totalP = 0;
+    for (auto& s : segs_) totalP = std::max<int64_t>(totalP, s.pOff + s.nP);
+    totalP = std::max<int64_t>(totalP, 1);
+    totalP_ = totalP;
+    labels_.reserve(totalP);
+    keys_.reserve(totalP);
+    members_.reserve(totalP);
+    covered_.reserve(totalP);
+    const int64_t slots = std::max<int64_t>(totalSlots_, 1);
+    chainOff_.reserve(slots);
+    chainEnd_.reserve(slots);
+    counts_.reserve(slots);
+    moved_.reserve(slots);
+    sums_.reserve((size_t)sStride_ * dim_);
+    sortWs_.reserve(sort_workspace_bytes(totalP, (int)std::max<int64_t>(totalSlots_, 2)));
+    check(hipMemcpyAsync(dN_.get(), &totalP, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
+    // decimate scratch: elemBase, groupOff/Cnt/Left, elemSlot, elemGroup, c1, c2, rowOff, alive
+    const int ns = std::max<int>((int)segs_.size(), 1);
+    const int ng = std::max<int>((int)groupBegin_.size(), 1);
+    decInts_.reserve((size_t)ns + 3 * ng + 5 * slots + (slots + 1) + scan_partials_count(std::max<int64_t>(slots + 1, ns)));
+    decI64_.reserve(2);
+}
+
+template <typename T>
+void Engine<T>::fillSentinels() {
+    launch_fill_sentinel<T>(S_.get(), sStride_, dim_, dSegs_.get(), dSegK_.get(), (int)segs_.size(), st_);
+    check(hipGetLastError(), "fill_sentinel");
+}
+
+// ------------------------------------------------------------------------------------
+// choose
+// ------------------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::choose() {
+    const int ns = (int)segs_.size();
+    if (ns == 0) return;
+    auto t0 = std::chrono::steady_clock::now();
+    constexpr int kLarge = 32768;
+    std::vector<int> large(ns);
+    int maxOwn = 0;
+    bool anyLarge = false;
+    for (int s = 0; s < ns; ++s) {
+        large[s] = segs_[s].nOwn > kLarge ? 1 : 0;
+        anyLarge |= large[s] != 0;
+        if (large[s]) maxOwn = std::max(maxOwn, segs_[s].nOwn);
+    }
+    DevBuf<int> dLarge;
+    dLarge.reserve(ns);
+    check(hipMemcpyAsync(dLarge.get(), large.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "large");
+    int zero = 0;
+    check(hipMemcpyAsync(flags_.get() + 2, &zero, sizeof(int), hipMemcpyHostToDevice, st_), "overflow");
+    ChooseArgs<T> a{P_.get(), pStride_, S_.get(), sStride_, dSegs_.get(), ns, dSegK_.get(), dSegR1Sqr_.get(),
+                    covered_.get(), chosenIdx_.get(), flags_.get() + 2, dLarge.get()};
+    // small segments: one block each, whole segment
+    check(launch_choose_block<T>(a, dim_, -1, 0, st_), "choose_block");
+    if (anyLarge) {
+        // chunk 0 sequentially, then filter + resolve per chunk
+        check(launch_choose_block<T>(a, dim_, 0, 0, st_), "choose_block");
+        std::vector<DevBuf<WorkItem>> bufs;
+        for (int c = 1; choose_chunk_start(c) < maxOwn; ++c) {
+            std::vector<WorkItem> items;
+            const int c0 = choose_chunk_start(c), c1 = choose_chunk_start(c + 1);
+            for (int s = 0; s < ns; ++s) {
+                if (!large[s]) continue;
+                const int e = std::min(c1, segs_[s].nOwn);
+                for (int st = c0; st < e; st += 1024) items.push_back({s, st, std::min(e, st + 1024), 0});
+            }
+            if (items.empty()) continue;
+            DevBuf<WorkItem> buf;
+            buf.reserve(items.size());
+            check(hipMemcpyAsync(buf.get(), items.data(), sizeof(WorkItem) * items.size(), hipMemcpyHostToDevice, st_),
+                  "items");
+            check(launch_choose_filter<T>(a, dim_, buf.get(), (int)items.size(), st_), "choose_filter");
+            check(launch_choose_block<T>(a, dim_, c, 1, st_), "choose_block");
+            check(hipStreamSynchronize(st_), "choose chunk");
+        }
+    }
+    fillSentinels();
+    int ovf = 0;
+    check(hipMemcpyAsync(&ovf, flags_.get() + 2, sizeof(int), hipMemcpyDeviceToHost, st_), "overflow");
+    check(hipStreamSynchronize(st_), "choose");
+    if (ovf) throw std::runtime_error("choose: chosen points exceed a segment's capacity");
+    timers.chooseMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------------------------
+// evolve
+// ------------------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::enqueueIteration(bool record) {
+    ScanArgs<T> sa{P_.get(), pStride_, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get()};
+    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, st_), "evolve_scan");
+    const int64_t totalP = totalP_;
+    check(stable_sort_by_key(labels_.get(), nullptr, dN_.get(), totalP, (int)std::max<int64_t>(totalSlots_, 2),
+                             keys_.get(), members_.get(), dM_.get(), sortWs_.get(), st_, flags_.get()),
+          "sort");
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, chainOff_.get(), totalSlots_,
+                       0, flags_.get());
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, chainEnd_.get(), totalSlots_,
+                       0, flags_.get());
+    check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), flags_.get(), st_),
+          "chain_bounds");
+    ShiftArgs<T> ha{P_.get(), pStride_, S_.get(), sStride_, totalSlots_, dSlotSeg_.get(), dSegs_.get(),
+                    dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), chainOff_.get(),
+                    chainEnd_.get(), flags_.get(), record ? counts_.get() : nullptr,
+                    record ? sums_.get() : nullptr, record ? moved_.get() : nullptr};
+    check(launch_chain_sum_shift<T>(ha, dim_, st_), "chain_sum_shift");
+}
+
+template <typename T>
+long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) {
+    const int ns = (int)segs_.size();
+    if (ns == 0) return 0;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> done(ns, 0), zeros(ns, 0);
+    int anyActive = 0;
+    for (int s = 0; s < ns; ++s) {
+        done[s] = (!active.empty() && !active[s]) ? 1 : 0;
+        anyActive |= !done[s];
+    }
+    if (!anyActive) return 0;
+    int f[2] = {0, 0};
+    check(hipMemcpyAsync(dSegDone_.get(), done.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
+    check(hipMemcpyAsync(dSegIter_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "iter");
+    check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
+    check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
+    long long iters = 0;
+    for (;;) {
+        const int batch = std::max(1, options.itersPerSync);
+        for (int b = 0; b < batch; ++b) {
+            enqueueIteration(false);
+            check(launch_evolve_finalize(ns, dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
+                                         flags_.get(), flags_.get() + 1, st_),
+                  "finalize");
+        }
+        check(hipMemcpyAsync(f, flags_.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "flags");
+        check(hipStreamSynchronize(st_), "evolve");
+        iters = f[1];
+        if (f[0]) break;
+        if (iters >= kEvolveGuard)
+            throw std::runtime_error("evolve did not converge within the iteration guard");
+    }
+    lastEvolveIters = iters;
+    timers.evolveIterations += iters;
+    timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return iters;
+}
+
+template <typename T>
+int Engine<T>::evolveIteration(std::vector<int>* labels, std::vector<int>* counts, std::vector<T>* sums,
+                               std::vector<unsigned short>* moved) {
+    const int ns = (int)segs_.size();
+    std::vector<int> zeros(ns, 0);
+    int f[2] = {0, 0};
+    check(hipMemcpyAsync(dSegDone_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
+    check(hipMemcpyAsync(dSegIter_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "iter");
+    check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
+    check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, counts_.get(), totalSlots_, 0,
+                       nullptr);
+    enqueueIteration(true);
+    std::vector<int> moved0(ns);
+    check(hipMemcpyAsync(moved0.data(), dSegMoved_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost, st_), "moved");
+    const int64_t totalP = segs_.empty() ? 0 : segs_.back().pOff + segs_.back().nP;
+    if (labels) {
+        labels->resize((size_t)totalP);
+        check(hipMemcpyAsync(labels->data(), labels_.get(), sizeof(int) * totalP, hipMemcpyDeviceToHost, st_), "labels");
+    }
+    if (counts) {
+        counts->resize((size_t)totalSlots_);
+        check(hipMemcpyAsync(counts->data(), counts_.get(), sizeof(int) * totalSlots_, hipMemcpyDeviceToHost, st_),
+              "counts");
+    }
+    if (sums) {
+        sums->resize((size_t)sStride_ * dim_);
+        check(hipMemcpyAsync(sums->data(), sums_.get(), sizeof(T) * sStride_ * dim_, hipMemcpyDeviceToHost, st_),
+              "sums");
+    }
+    if (moved) {
+        moved->resize((size_t)totalSlots_);
+        check(hipMemcpyAsync(moved->data(), moved_.get(), sizeof(unsigned short) * totalSlots_, hipMemcpyDeviceToHost,
+                             st_),
+              "moved");
+    }
+    check(hipStreamSynchronize(st_), "evolveIteration");
+    int any = 0;
+    for (int v : moved0) any |= v;
+    return any;
+}
+
+// ------------------------------------------------------------------------------------
+// decimate
+// ------------------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::decimate(bool noCopyRule, std::vector<unsigned char>* keep) {
+    const int ns = (int)segs_.size();
+    if (ns == 0) return;
+    auto t0 = std::chrono::steady_clock::now();
+    const int ng = (int)groupBegin_.size();
+    const int64_t slots = std::max<int64_t>(totalSlots_, 1);
+    int* p = decInts_.get();
+    DecArgs a{};
+    a.segs = dSegs_.get();
+    a.numSegs = ns;
+    a.segK = dSegK_.get();
+    a.groupSegBegin = dGroupBegin_.get();
+    a.groupSegEnd = dGroupEnd_.get();
+    a.numGroups = ng;
+    a.noCopyRule = noCopyRule ? 1 : 0;
+    a.elemBase = p; p += ns;
+    a.groupOff = p; p += ng;
+    a.groupCnt = p; p += ng;
+    a.groupLeft = p; p += ng;
+    a.elemSlot = p; p += slots;
+    a.elemGroup = p; p += slots;
+    a.c1 = p; p += slots;
+    a.c2 = p; p += slots;
+    a.alive = p; p += slots;
+    a.rowOff = p; p += slots + 1;
+    decWs_.partials = p;
+    a.totalElems = decI64_.get();
+    decWs_.nbrTotal = decI64_.get() + 1;
+    a.segSkip = segSkip_;
+    DevBuf<unsigned char> keepBuf;
+    if (keep) keepBuf.reserve(slots);
+    check(decimate_groups<T>(a, S_.get(), sStride_, dim_, dGroupR2Sqr_.get(), chosenIdx_.get(),
+                             keep ? keepBuf.get() : nullptr, totalSlots_, decWs_, st_),
+          "decimate");
+    if (keep) {
+        keep->resize((size_t)totalSlots_);
+        check(hipMemcpyAsync(keep->data(), keepBuf.get(), totalSlots_, hipMemcpyDeviceToHost, st_), "keep");
+    }
+    check(hipStreamSynchronize(st_), "decimate");
+    timers.decimateMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------------------------
+// drivers
+// ------------------------------------------------------------------------------------
+template <typename T>
+std::vector<int> Engine<T>::countsHost() {
+    std::vector<int> k(segs_.size());
+    if (!k.empty()) {
+        check(hipMemcpyAsync(k.data(), dSegK_.get(), sizeof(int) * k.size(), hipMemcpyDeviceToHost, st_), "K");
+        check(hipStreamSynchronize(st_), "K");
+    }
+    return k;
+}
+
+template <typename T>
+int64_t Engine<T>::totalCount() {
+    auto k = countsHost();
+    return std::accumulate(k.begin(), k.end(), int64_t(0));
+}
+
+template <typename T>
+void Engine<T>::ridgeDetectionLocal(const RunOptions& opt) {
+    options = opt;
+    auto t0 = std::chrono::steady_clock::now();
+    const int ns = (int)segs_.size();
+    choose();
+    std::vector<int> K = countsHost();
+    std::vector<char> active(ns);
+    std::vector<int> skip(ns);
+    for (int s = 0; s < ns; ++s) active[s] = K[s] > 0;
+    DevBuf<int> dSkip;
+    dSkip.reserve(std::max(ns, 1));
+    long long rounds = 0;
+    for (;;) {
+        bool any = false;
+        for (int s = 0; s < ns; ++s) any |= active[s] != 0;
+        if (!any) break;
+        if (opt.maxRounds > 0 && rounds >= opt.maxRounds) break;
+        evolve(opt.maxEvolveIters, active);
+        for (int s = 0; s < ns; ++s) skip[s] = active[s] ? 0 : 1;
+        check(hipMemcpyAsync(dSkip.get(), skip.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "skip");
+        segSkip_ = dSkip.get();
+        decimate(false);
+        segSkip_ = nullptr;
+        std::vector<int> K2 = countsHost();
+        for (int s = 0; s < ns; ++s)
+            if (active[s] && K2[s] == K[s]) active[s] = 0;   // count stable: this segment is done
+        K = K2;
+        ++rounds;
+    }
+    timers.rounds += rounds;
+    timers.wholeMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T>
+void Engine<T>::ridgeDetectionMixed(const RunOptions& opt) {
+    options = opt;
+    auto t0 = std::chrono::steady_clock::now();
+    const int ns = (int)segs_.size();
+    choose();
+    std::vector<int> K = countsHost();
+    int64_t total = std::accumulate(K.begin(), K.end(), int64_t(0));
+    int64_t oldTotal = 0;
+    std::vector<char> active(ns, 1);
+    long long rounds = 0;
+    while (oldTotal != total) {
+        if (opt.maxRounds > 0 && rounds >= opt.maxRounds) break;
+        oldTotal = total;
+        evolve(opt.maxEvolveIters, active);
+        decimate(true);
+        std::vector<int> K2 = countsHost();
+        total = std::accumulate(K2.begin(), K2.end(), int64_t(0));
+        // a leaf whose chosen set did not change is a fixed point of an uncapped evolve
+        for (int s = 0; s < ns; ++s) active[s] = (opt.maxEvolveIters > 0 || K2[s] != K[s]) ? 1 : 0;
+        K = K2;
+        ++rounds;
+    }
+    timers.rounds += rounds;
+    timers.wholeMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------------------------
+// results
+// ------------------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::getChosen(T* out, int layout, bool onDevice, int64_t outStride) {
+    const int ns = (int)segs_.size();
+    std::vector<int> K = countsHost();
+    std::vector<int64_t> off(ns);
+    int64_t total = 0;
+    for (int s = 0; s < ns; ++s) {
+        off[s] = total;
+        total += K[s];
+    }
+    if (total == 0) return;
+    if (layout == LAYOUT_ROW_MAJOR && outStride <= 0) outStride = dim_;
+    if (layout != LAYOUT_ROW_MAJOR && outStride <= 0) outStride = total;
+    DevBuf<int64_t> dOff;
+    dOff.reserve(ns);
+    check(hipMemcpyAsync(dOff.get(), off.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st_), "off");
+    const size_t bytes = sizeof(T) * (layout == LAYOUT_ROW_MAJOR ? total * outStride : outStride * dim_);
+    DevBuf<T> staging;
+    T* dst = out;
+    if (!onDevice) dst = staging.reserve(bytes / sizeof(T));
+    hipLaunchKernelGGL((gather_chosen_kernel<T>), dim3(ns), dim3(256), 0, st_, S_.get(), sStride_, dSegs_.get(),
+                       dSegK_.get(), dOff.get(), dim_, dst, layout == LAYOUT_ROW_MAJOR ? 0 : 1, outStride);
+    check(hipGetLastError(), "gather");
+    if (!onDevice) check(hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, st_), "download");
+    check(hipStreamSynchronize(st_), "getChosen");
+}
+
+template <typename T>
+std::vector<int> Engine<T>::chosenIndicesHost(int seg) {
+    std::vector<int> K = countsHost();
+    std::vector<int> idx((size_t)K.at(seg));
+    if (!idx.empty()) {
+        check(hipMemcpyAsync(idx.data(), chosenIdx_.get() + segs_[seg].sOff, sizeof(int) * idx.size(),
+                             hipMemcpyDeviceToHost, st_),
+              "idx");
+        check(hipStreamSynchronize(st_), "idx");
+    }
+    return idx;
+}
+
+template <typename T>
+void Engine<T>::setChosen(int seg, const T* S, int64_t K, int layout) {
+    if (seg < 0 || seg >= (int)segs_.size()) throw std::invalid_argument("segment out of range");
+    if (K < 0 || K > segs_[seg].sCap) throw std::invalid_argument("too many chosen points for the segment");
+    std::vector<T> soa((size_t)K * dim_);
+    for (int64_t k = 0; k < K; ++k)
+        for (int d = 0; d < dim_; ++d)
+            soa[(size_t)d * K + k] = layout == LAYOUT_ROW_MAJOR ? S[k * dim_ + d] : S[(size_t)d * K + k];
+    if (K > 0)
+        check(hipMemcpy2DAsync(S_.get() + segs_[seg].sOff, sizeof(T) * sStride_, soa.data(), sizeof(T) * K,
+                               sizeof(T) * K, dim_, hipMemcpyHostToDevice, st_),
+              "setChosen");
+    int k32 = (int)K;
+    check(hipMemcpyAsync(dSegK_.get() + seg, &k32, sizeof(int), hipMemcpyHostToDevice, st_), "K");
+    std::vector<int> idx((size_t)std::max<int64_t>(K, 1));
+    std::iota(idx.begin(), idx.end(), 0);
+    if (K > 0)
+        check(hipMemcpyAsync(chosenIdx_.get() + segs_[seg].sOff, idx.data(), sizeof(int) * K, hipMemcpyHostToDevice,
+                             st_),
+              "idx");
+    fillSentinels();
+    check(hipStreamSynchronize(st_), "setChosen");
+}
+
+template class Engine<float>;
+template class Engine<double>;
+
+}  // namespace rdhip

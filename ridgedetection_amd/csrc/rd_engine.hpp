@@ -1,0 +1,165 @@
+// rd_engine.hpp -- host-side engine: device state and phase orchestration for one
+// "segment set" (a whole cloud, the leaves of a tile tree, or a batch of clouds).
+#pragma once
+
+#include "rd_choose.hpp"
+#include "rd_common.hpp"
+#include "rd_decimate.hpp"
+#include "rd_evolve.hpp"
+#include "rd_sort.hpp"
+
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace rdhip {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string& where)
+        : std::runtime_error(where + ": " + hipGetErrorString(c)), code(c) {}
+};
+
+inline void check(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw HipError(e, where);
+}
+
+// Grow-only device buffer.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T* reserve(size_t count) {
+        if (count <= n && p) return p;
+        release();
+        size_t c = count ? count : 1;
+        check(hipMalloc(&p, sizeof(T) * c), "hipMalloc");
+        n = c;
+        return p;
+    }
+    T* get() const { return p; }
+};
+
+enum Layout { LAYOUT_ROW_MAJOR = 0, LAYOUT_COL_MAJOR = 1, LAYOUT_SOA = 2 };
+
+struct Timers {
+    double wholeMs = 0, buildMs = 0, chooseMs = 0, evolveMs = 0, decimateMs = 0, refineMs = 0;
+    long long evolveIterations = 0, rounds = 0;
+};
+
+struct RunOptions {
+    long long maxEvolveIters = 0;   // per evolve call; 0 = until convergence (reference)
+    long long maxRounds = 0;        // 0 = until the count is stable (reference)
+    int ppt = 4;                    // samples per lane in the evolve scan
+    int itersPerSync = 8;           // evolve iterations enqueued between convergence checks
+};
+
+template <typename T>
+class Engine {
+public:
+    Engine(int dim, hipStream_t stream);
+    ~Engine();
+
+    int dim() const { return dim_; }
+    hipStream_t stream() const { return st_; }
+
+    // -- points --------------------------------------------------------------------
+    // Uploads n points (host or device pointer) given in ROW_MAJOR, COL_MAJOR (stride =
+    // elements between rows of the same coordinate) or SOA layout into the SoA buffer.
+    void setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice);
+    int64_t numPoints() const { return nPoints_; }
+    const T* devicePoints() const { return P_.get(); }
+    int64_t pointStride() const { return pStride_; }
+
+    // -- segments ------------------------------------------------------------------
+    // Defines the segments over the SoA buffer (pOff/nP/nOwn/group must be set; sOff and
+    // sCap are assigned here from capacities).  Group g = consecutive segments with
+    // group == g.  r1/r2 are per segment.
+    void setSegments(const std::vector<SegDesc>& segs, const std::vector<int64_t>& capacities,
+                     const std::vector<T>& r1, const std::vector<T>& r2);
+    void setSingleSegment(T r1, T r2);
+    // Replaces the radii of the current segments without touching their chosen points.
+    void setRadii(const std::vector<T>& r1, const std::vector<T>& r2);
+    int numSegments() const { return (int)segs_.size(); }
+
+    // -- phases --------------------------------------------------------------------
+    void choose();
+    // Runs evolve iterations on every segment whose flag in `active` is set (all when
+    // empty) until each converges or reaches maxIter.  Returns the number of iterations.
+    long long evolve(long long maxIter, const std::vector<char>& active = {});
+    // One evolve iteration on all segments, recording labels/counts/sums/moved on host.
+    int evolveIteration(std::vector<int>* labels, std::vector<int>* counts, std::vector<T>* sums,
+                        std::vector<unsigned short>* moved);
+    void decimate(bool noCopyRule, std::vector<unsigned char>* keep = nullptr);
+
+    // -- drivers -------------------------------------------------------------------
+    // Brute force over every segment independently (each its own group): choose, then
+    // rounds of evolve + decimate until each segment's count is stable.
+    void ridgeDetectionLocal(const RunOptions& opt);
+    // Mixed: choose per segment, then rounds of evolve per segment + one decimateNoCopy over
+    // all segments (single group) until the total count is stable.
+    void ridgeDetectionMixed(const RunOptions& opt);
+
+    // -- results -------------------------------------------------------------------
+    std::vector<int> countsHost();
+    int64_t totalCount();
+    // Chosen points of all segments in segment order, into `out` with `layout`.
+    void getChosen(T* out, int layout, bool onDevice, int64_t outStride = 0);
+    std::vector<int> chosenIndicesHost(int seg);
+    void setChosen(int seg, const T* S, int64_t K, int layout);   // host ROW_MAJOR/SOA input
+
+    Timers timers;
+    RunOptions options;
+    int64_t lastEvolveIters = 0;
+
+private:
+    int dim_;
+    hipStream_t st_;
+    // points
+    DevBuf<T> P_;
+    int64_t nPoints_ = 0, pStride_ = 0;
+    // segments
+    std::vector<SegDesc> segs_;
+    std::vector<int> groupBegin_, groupEnd_;
+    std::vector<T> r1_, r2_;
+    DevBuf<SegDesc> dSegs_;
+    DevBuf<int> dSegK_, dSegDone_, dSegIter_, dSegMoved_, dGroupBegin_, dGroupEnd_, dSlotSeg_;
+    DevBuf<T> dSegR1Sqr_, dGroupR2Sqr_;
+    DevBuf<WorkItem> dItems_;
+    int numItems_ = 0;
+    int64_t totalSlots_ = 0, sStride_ = 0, totalP_ = 0;
+    DevBuf<T> S_;
+    DevBuf<int> chosenIdx_;
+    // evolve scratch
+    DevBuf<int> labels_, keys_, members_, chainOff_, chainEnd_, counts_;
+    DevBuf<T> sums_;
+    DevBuf<unsigned short> moved_;
+    DevBuf<unsigned char> sortWs_;
+    DevBuf<int64_t> dN_, dM_;
+    DevBuf<int> flags_;   // [0] allDone, [1] totalIters, [2] overflow
+    // choose scratch
+    DevBuf<unsigned char> covered_;
+    std::vector<DevBuf<WorkItem>*> chooseItems_;
+    // decimate scratch
+    DevBuf<int> decInts_;
+    DevBuf<int64_t> decI64_;
+    DecWorkspace decWs_;
+    const int* segSkip_ = nullptr;   // optional per-segment skip mask for decimate
+
+    void buildWorkItems();
+    void ensureEvolveScratch();
+    void enqueueIteration(bool record);
+    void fillSentinels();
+};
+
+}  // namespace rdhip

@@ -1,0 +1,60 @@
+// rd_evolve.hpp -- launchers for the evolve iteration kernels (rd_evolve.hip).
+#pragma once
+
+#include "rd_common.hpp"
+
+#include <algorithm>
+
+namespace rdhip {
+
+constexpr int kScanThreads = 256;
+
+template <typename T>
+struct ScanArgs {
+    const T* P;
+    int64_t pStride;
+    const T* S;
+    int64_t sStride;
+    const WorkItem* items;
+    const SegDesc* segs;
+    const int* segK;
+    const int* segDone;
+    const T* segRSqr;
+    const int* allDone;
+    int* labels;   // per sample: chosen slot, or -1 when outside every r-ball
+};
+
+template <typename T>
+struct ShiftArgs {
+    const T* P;
+    int64_t pStride;
+    T* S;
+    int64_t sStride;
+    int64_t totalSlots;
+    const int* slotSeg;     // owning segment of every chosen slot (-1: padding)
+    const SegDesc* segs;
+    const int* segK;
+    const int* segDone;
+    int* segMoved;
+    const int* members;     // sample indices sorted by (label, sample)
+    const int* chainOff;
+    const int* chainEnd;
+    const int* allDone;
+    int* outCounts;         // optional: members per slot
+    T* outSums;             // optional: SoA coordinate sums per slot
+    unsigned short* outMoved;  // optional: per-slot bit mask of moved coordinates
+};
+
+template <typename T>
+hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, hipStream_t st);
+template <typename T>
+hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, hipStream_t st);
+hipError_t launch_chain_bounds(const int* keys, const int64_t* dM, int64_t maxM, int* chainOff,
+                               int* chainEnd, const int* allDone, hipStream_t st);
+hipError_t launch_evolve_finalize(int numSegs, int* segDone, int* segMoved, int* segIter, int maxIter,
+                                  int* allDone, int* totalIters, hipStream_t st);
+template <typename T>
+hipError_t launch_fill_sentinel(T* S, int64_t sStride, int dim, const SegDesc* segs, const int* segK,
+                                int numSegs, hipStream_t st);
+
+}  // namespace rdhip

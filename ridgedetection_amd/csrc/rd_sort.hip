@@ -1,0 +1,271 @@
+// rd_sort.hip -- stable sort of (key, value) int32 pairs by key on gfx950.
+//
+// LSD radix sort with digits of at most 11 bits.  Each pass is three launches:
+//   hist    -- per-block digit histogram (LDS atomics; order-free integer counts)
+//   scan    -- exclusive scan of the digit-major histogram [digit][block] (3 kernels)
+//   scatter -- per-block stable ranking: every wave ranks its 64 keys with a ballot
+//              match (one ballot per digit bit), carries per-digit counters across its
+//              chunks in LDS, and the four waves of a block are stitched together by a
+//              per-digit prefix, so equal keys keep their input order.
+// Negative keys are dropped by the first pass, which also writes the count of kept
+// pairs.  Used to list the members of every chosen point in point order (evolve) and to
+// bin points by tile (tree build).
+#include "rd_sort.hpp"
+
+#include <algorithm>
+
+namespace rdhip {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / kWave;
+constexpr int kChunks = 16;                                  // 64-key chunks per wave
+constexpr int kBlockElems = kWavesPerBlock * kChunks * kWave;  // 4096
+constexpr int kMaxDigitBits = 11;
+constexpr int kScanElems = 4096;                              // entries per scan block
+
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const int* __restrict__ keys,
+                                                              const int64_t* __restrict__ nPtr,
+                                                              int shift, int bits, int numBlocks,
+                                                              int* __restrict__ ghist,
+                                                              const int* __restrict__ skip) {
+    __shared__ int h[1 << kMaxDigitBits];
+    if (skip && *skip) return;
+    const int nb = 1 << bits;
+    for (int v = threadIdx.x; v < nb; v += kThreads) h[v] = 0;
+    __syncthreads();
+    const int64_t n = *nPtr;
+    const int64_t b0 = (int64_t)blockIdx.x * kBlockElems;
+    const int mask = nb - 1;
+    for (int e = threadIdx.x; e < kBlockElems; e += kThreads) {
+        int64_t i = b0 + e;
+        if (i < n) {
+            int k = keys[i];
+            if (k >= 0) atomicAdd(&h[(k >> shift) & mask], 1);
+        }
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < nb; v += kThreads) ghist[(int64_t)v * numBlocks + blockIdx.x] = h[v];
+}
+
+// Exclusive scan, phase 1: per-block totals of kScanElems entries.
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const int* __restrict__ a, int64_t L,
+                                                               int* __restrict__ partials,
+                                                               const int* __restrict__ skip) {
+    __shared__ int red[kThreads];
+    if (skip && *skip) return;
+    const int64_t b0 = (int64_t)blockIdx.x * kScanElems;
+    int s = 0;
+    for (int e = threadIdx.x; e < kScanElems; e += kThreads) {
+        int64_t i = b0 + e;
+        if (i < L) s += a[i];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// Phase 2: one block scans the partials in place and writes the grand total.
+__global__ __launch_bounds__(1024) void scan_partials_kernel(int* __restrict__ partials, int np,
+                                                             int64_t* __restrict__ total,
+                                                             const int* __restrict__ skip) {
+    __shared__ int buf[1024];
+    if (skip && *skip) return;
+    int carry = 0;
+    for (int b0 = 0; b0 < np; b0 += 1024) {
+        int i = b0 + threadIdx.x;
+        int v = i < np ? partials[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < np) partials[i] = carry + buf[threadIdx.x] - v;
+        int blockTotal = buf[1023];
+        __syncthreads();
+        carry += blockTotal;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// Phase 3: scan inside each block (16 consecutive entries per thread) plus its offset.
+__global__ __launch_bounds__(kThreads) void scan_apply_kernel(int* __restrict__ a, int64_t L,
+                                                              const int* __restrict__ partials,
+                                                              const int* __restrict__ skip) {
+    __shared__ int buf[kThreads];
+    if (skip && *skip) return;
+    constexpr int per = kScanElems / kThreads;
+    const int64_t b0 = (int64_t)blockIdx.x * kScanElems + (int64_t)threadIdx.x * per;
+    int v[per];
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        int64_t i = b0 + j;
+        v[j] = i < L ? a[i] : 0;
+        s += v[j];
+    }
+    buf[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < kThreads; o <<= 1) {
+        int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+        __syncthreads();
+        buf[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = partials[blockIdx.x] + buf[threadIdx.x] - s;
+#pragma unroll
+    for (int j = 0; j < per; ++j) {
+        int64_t i = b0 + j;
+        if (i < L) a[i] = run;
+        run += v[j];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
+    const int* __restrict__ keysIn, const int* __restrict__ valsIn, const int64_t* __restrict__ nPtr,
+    int shift, int bits, int numBlocks, const int* __restrict__ ghistScanned,
+    int* __restrict__ keysOut, int* __restrict__ valsOut, const int* __restrict__ skip) {
+    __shared__ int cnt[kWavesPerBlock][1 << kMaxDigitBits];
+    if (skip && *skip) return;
+    const int nb = 1 << bits;
+    const int mask = nb - 1;
+    for (int v = threadIdx.x; v < nb; v += kThreads)
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) cnt[w][v] = 0;
+    __syncthreads();
+
+    const int64_t n = *nPtr;
+    const int wave = threadIdx.x / kWave;
+    const int lane = lane_id();
+    const uint64_t below = lanes_below();
+    const int64_t w0 = (int64_t)blockIdx.x * kBlockElems + (int64_t)wave * (kChunks * kWave);
+
+    int key[kChunks], pos[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        int64_t i = w0 + c * kWave + lane;
+        int k = (i < n) ? keysIn[i] : -1;
+        key[c] = k;
+        const bool valid = k >= 0;
+        const int dig = valid ? ((k >> shift) & mask) : 0;
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (dig >> b) & 1;
+            const uint64_t bb = __ballot(valid && bit);
+            m &= bit ? bb : ~bb;
+        }
+        int p = 0;
+        if (valid) {
+            p = cnt[wave][dig] + __popcll(m & below);
+        }
+        // the lowest lane of each group of equal digits advances that digit's counter;
+        // within a wave LDS operations complete in issue order, so the reads above
+        // observe the counter before this update.
+        if (valid && (m & below) == 0) cnt[wave][dig] += __popcll(m);
+        pos[c] = p;
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < nb; v += kThreads) {
+        int run = ghistScanned[(int64_t)v * numBlocks + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            int t = cnt[w][v];
+            cnt[w][v] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        int k = key[c];
+        if (k >= 0) {
+            int64_t i = w0 + c * kWave + lane;
+            int dig = (k >> shift) & mask;
+            int dst = cnt[wave][dig] + pos[c];
+            keysOut[dst] = k;
+            valsOut[dst] = valsIn ? valsIn[i] : (int)i;
+        }
+    }
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+int64_t scan_partials_count(int64_t L) { return cdiv(std::max<int64_t>(L, 1), kScanElems); }
+
+hipError_t exclusive_scan_inplace(int* a, int64_t L, int* partials, int64_t* total, hipStream_t stream) {
+    if (L <= 0) return hipSuccess;
+    const int64_t scanBlocks = cdiv(L, kScanElems);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream, a, L, partials,
+                       nullptr);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, stream, partials, (int)scanBlocks, total,
+                       nullptr);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream, a, L, partials,
+                       nullptr);
+    return hipGetLastError();
+}
+
+size_t sort_workspace_bytes(int64_t maxN, int numKeys) {
+    int bits = 1;
+    while ((int64_t(1) << bits) < numKeys) ++bits;
+    int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    int w = (bits + passes - 1) / passes;
+    int64_t blocks = cdiv(std::max<int64_t>(maxN, 1), kBlockElems);
+    int64_t L = (int64_t(1) << w) * blocks;
+    int64_t scanBlocks = cdiv(L, kScanElems);
+    // ghist + partials + temp keys/vals + count
+    return sizeof(int) * (size_t)(L + scanBlocks + 2 * std::max<int64_t>(maxN, 1)) + 64;
+}
+
+hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
+                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
+                              hipStream_t stream, const int* skip) {
+    if (maxN <= 0) return hipSuccess;
+    int bits = 1;
+    while ((int64_t(1) << bits) < numKeys) ++bits;
+    const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int w = (bits + passes - 1) / passes;
+    const int64_t blocks = cdiv(maxN, kBlockElems);
+    const int64_t L = (int64_t(1) << w) * blocks;
+    const int64_t scanBlocks = cdiv(L, kScanElems);
+    int* ghist = static_cast<int*>(workspace);
+    int* partials = ghist + L;
+    int* tmpK = partials + scanBlocks;
+    int* tmpV = tmpK + maxN;
+
+    const int* inK = keysIn;
+    const int* inV = valsIn;
+    const int64_t* n = dN;
+    for (int p = 0; p < passes; ++p) {
+        // ping-pong so that the last pass lands in keysOut/valsOut
+        const bool toOut = ((passes - 1 - p) % 2) == 0;
+        int* oK = toOut ? keysOut : tmpK;
+        int* oV = toOut ? valsOut : tmpV;
+        const int shift = p * w;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK, n,
+                           shift, w, (int)blocks, ghist, skip);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream,
+                           ghist, L, partials, skip);
+        hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, stream, partials,
+                           (int)scanBlocks, p == 0 ? dM : nullptr, skip);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream,
+                           ghist, L, partials, skip);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
+                           inV, n, shift, w, (int)blocks, ghist, oK, oV, skip);
+        inK = oK;
+        inV = oV;
+        n = dM;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rdhip

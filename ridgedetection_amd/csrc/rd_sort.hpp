@@ -1,0 +1,25 @@
+// rd_sort.hpp -- stable device sort of int32 (key, value) pairs by key.
+#pragma once
+
+#include "rd_common.hpp"
+
+namespace rdhip {
+
+// Workspace needed by stable_sort_by_key for up to maxN pairs with keys in [0, numKeys).
+size_t sort_workspace_bytes(int64_t maxN, int numKeys);
+
+// Sorts the *dN pairs (keysIn[i], valsIn ? valsIn[i] : i) by key, keeping input order
+// among equal keys; pairs with a negative key are dropped.  The number of kept pairs is
+// written to *dM.  All counts live on the device, so the call is stream-ordered and
+// capturable.  maxN bounds *dN and sizes the grids.  When `skip` is not null and *skip
+// is set when the kernels run, they return at once (used by converged evolve loops).
+hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
+                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
+                              hipStream_t stream, const int* skip = nullptr);
+
+// Exclusive prefix sum of a[0..L) in place; `partials` needs scan_partials_count(L) ints;
+// the grand total is written to *total when total is not null.
+int64_t scan_partials_count(int64_t L);
+hipError_t exclusive_scan_inplace(int* a, int64_t L, int* partials, int64_t* total, hipStream_t stream);
+
+}  // namespace rdhip

@@ -1,0 +1,112 @@
+"""GPU parity of the brute-force path through the C ABI, bit-exact against the CPU oracle
+(the restatement pinned to the reference by tests/test_oracle_golden.py)."""
+
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from ridgedetection_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+BRUTE = sorted(glob.glob(os.path.join(GOLDEN, "brute_*.npz")))
+
+
+def _name(p):
+    return os.path.basename(p)[:-4]
+
+
+def _ctx(P):
+    from ridgedetection_amd._lib import Context
+    c = Context(P.shape[1], P.dtype)
+    c.set_points(P)
+    return c
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("path", BRUTE, ids=_name)
+def test_phases_match_golden(oracle, path):
+    g = np.load(path)
+    P, r1, r2 = g["P"], float(g["r1"]), float(g["r2"])
+    _say(f"case {_name(path)} n={len(P)} d={P.shape[1]} {P.dtype}")
+    with _ctx(P) as c:
+        S0, idx = c.choose(r1)
+        np.testing.assert_array_equal(idx, g["choose_idx"])
+        np.testing.assert_array_equal(S0, g["choose_S"])
+        S1, flag, counts, sums, labels, moved = c.evolve_iteration(r1)
+        oS1, oflag, ocounts, osums, olabels, omoved = oracle.evolve_iter(P, S0, r1)
+        np.testing.assert_array_equal(labels, olabels)
+        np.testing.assert_array_equal(counts, g["iter_counts"])
+        np.testing.assert_array_equal(sums, g["iter_sums"])
+        np.testing.assert_array_equal(S1, g["iter_S"])
+        assert flag == int(g["iter_flag"])
+        mv = (moved[:, None] >> np.arange(P.shape[1])[None, :]) & 1
+        np.testing.assert_array_equal(mv.astype(np.uint8), omoved)
+        c.set_chosen(S0)
+        SE, it = c.evolve(r1)
+        np.testing.assert_array_equal(SE, g["evolve_S"])
+        SD, keep = c.decimate(r2)
+        np.testing.assert_array_equal(SD, g["decimate_S"])
+        np.testing.assert_array_equal(np.nonzero(keep)[0], g["decimate_kept"])
+        SR = c.ridge_detection(r1, r2)
+        np.testing.assert_array_equal(SR, g["rd_S"])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_large_cloud_choose_and_iteration(oracle, dt):
+    """>32768 points takes the chunked multi-CU choose path."""
+    P = synth.helix3d(120000, seed=21, dtype=dt)
+    _say(f"case helix120k {np.dtype(dt).name}")
+    with _ctx(P) as c:
+        S0, idx = c.choose(0.2)
+        oS0, oidx = oracle.choose(P, 0.2)
+        np.testing.assert_array_equal(idx, oidx)
+        np.testing.assert_array_equal(S0, oS0)
+        S1, flag, counts, sums, labels, moved = c.evolve_iteration(0.2)
+        oS1, oflag, ocounts, osums, olabels, _ = oracle.evolve_iter(P, S0, 0.2)
+        np.testing.assert_array_equal(labels, olabels)
+        np.testing.assert_array_equal(counts, ocounts)
+        np.testing.assert_array_equal(sums, osums)
+        np.testing.assert_array_equal(S1, oS1)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 4, 5, 7, 12])
+def test_dims(oracle, dim):
+    g = np.random.Generator(np.random.PCG64(dim))
+    P = synth.segment_chain(6000, dim, seed=dim, sigma=0.05)
+    r1 = 0.15 + 0.05 * dim
+    _say(f"case segment_chain d={dim}")
+    with _ctx(P) as c:
+        SR = c.ridge_detection(r1, 2 * r1)
+    oSR, _ = oracle.ridge_detection(P, r1, 2 * r1)
+    np.testing.assert_array_equal(SR, oSR)
+
+
+def test_c1_spiral_capped(oracle):
+    """C1: 20k-point 2-D spiral, r=0.45, max 100 evolve iterations per round."""
+    P = synth.spiral2d(20000, seed=1)
+    _say("case C1 spiral20k capped")
+    with _ctx(P) as c:
+        SR = c.ridge_detection(0.45, 0.9, max_evolve_iters=100)
+        t = c.timers()
+    oSR, rounds = oracle.ridge_detection(P, 0.45, 0.9, max_evolve_iter=100)
+    np.testing.assert_array_equal(SR, oSR)
+    assert t["rounds"] == rounds
+
+
+def test_decimate_nocopy_rule(oracle):
+    g = np.load(os.path.join(GOLDEN, "nocopy_spiral_f32.npz"))
+    S = g["S"]
+    P = synth.spiral2d(2000, seed=3)
+    _say("case decimate no-copy")
+    with _ctx(P) as c:
+        c.set_chosen(S)
+        SD, keep = c.decimate(float(g["r2"]), rule=1)
+    np.testing.assert_array_equal(keep, g["alive"])
+    assert len(SD) == int(g["count"])
