@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the ridge-detection hot path on one MI355X.
+
+Headline metric: evolve pair-evaluations per second at 1M 3-D points (the helix workload:
+t ~ U[0, 10 pi], (cos t, sin t, 0.15 t) + N(0, 0.05^2), r = 0.2, FP32).  A step is one
+complete evolve iteration over the cloud: nearest-chosen scan, ordering of the members,
+ordered centroid sums and the shift, i.e. N x K pair evaluations with K chosen points.
+The chosen set is the greedy choose at r on the same cloud.  Inputs are resident in HBM
+when the timed region starts.
+
+Also reported on the same JSON line:
+  roofline     -- the scan kernel (dominant) against the FP32 VALU peak, with its average
+                  duration measured by HIP events on the library's stream;
+  cpu_baseline -- the reference CPU evolve pass (OpenMP build of the reference headers, or
+                  the oracle's OpenMP restatement when that build is absent) on the host;
+  secondary    -- per-iteration device time and the other phases of the run.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--dtype f32|f64] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector peak (dense)
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(P, S, r, budget_s=20.0):
+    """Times the reference's OpenMP evolve pass (calculateClosestSphereCenter_omp +
+    shiftTowardMassCenter_omp) over the same cloud and chosen set."""
+    from oracle.oracle import Oracle, available
+    kind = "reference" if available("ref_omp") else "port"
+    o = Oracle("ref_omp" if kind == "reference" else "oracle_omp")
+    threads = int(o.lib.ref_openmp_threads() if kind == "reference" else o.lib.oracle_num_threads())
+    n, k = len(P), len(S)
+    # bounded sample: as many points as fit the time budget, estimated from a small pass
+    m = min(n, 20000)
+    t0 = time.perf_counter()
+    o.evolve_omp_iter(P[:m], S, r)
+    rate = m * k / max(time.perf_counter() - t0, 1e-6)
+    m = int(min(n, max(20000, rate * budget_s / 3)))
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.evolve_omp_iter(P[:m], S, r)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s / 2 or reps >= 3:
+            break
+    value = reps * m * k / el
+    return {"value": value, "unit": "pair-evals/s", "cores": threads, "kind": kind,
+            "sample": f"{reps} evolve pass(es) of {m} of the {n} points x all {k} chosen points, "
+                      f"{el:.1f} s on {threads} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--ppt", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context, OPT_PROFILE, OPT_PPT
+
+    dt = np.float32 if args.dtype == "f32" else np.float64
+    r = 0.2
+    d = 3
+    log(f"bench: generating helix n={args.n} {args.dtype}")
+    P = synth.helix3d(args.n, seed=2024, sigma=0.05, turns=5.0, pitch=0.15, dtype=dt)
+    ctx = Context(d, dt)
+    ctx.set_points(P)
+    S0, _ = ctx.choose(r)
+    K = len(S0)
+    log(f"bench: K={K} chosen points")
+    ctx.set_option(OPT_PPT, args.ppt)
+    ctx.set_option(OPT_PROFILE, 1)
+    ctx.evolve_fixed(r, args.warmup)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    sync()
+    t_before = ctx.timers()
+    t0 = time.perf_counter()
+    ctx.evolve_fixed(r, args.steps)
+    sync()
+    el = time.perf_counter() - t0
+    t_after = ctx.timers()
+    log(f"bench: {args.steps} iterations in {el * 1e3:.2f} ms")
+
+    n = args.n
+    pairs = n * K
+    value = pairs * args.steps / el
+    launches = t_after["scan_launches"] - t_before["scan_launches"]
+    scan_ms = (t_after["scan_kernel_ms"] - t_before["scan_kernel_ms"]) / max(launches, 1)
+    iter_ms = (t_after["iteration_ms"] - t_before["iteration_ms"]) / max(launches, 1)
+    flops = n * K * 3 * d + n + n * d            # reference convention, per scan launch
+    achieved = flops / (scan_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if dt == np.float32 else FP64_PEAK_TFLOPS
+
+    # the rest of the path, for context: full brute-force ridge detection on the cloud
+    secondary = {
+        "chosen_points": K,
+        "scan_kernel_ms": round(scan_ms, 4),
+        "evolve_iteration_device_ms": round(iter_ms, 4),
+        "scan_pair_evals_per_s": pairs / (scan_ms * 1e-3),
+        "scan_flops_per_launch": flops,
+    }
+    result = {
+        "metric": "evolve pair-evals/s at 1M 3-D pts",
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded helix, resident in HBM)",
+        "config": {"workload": "helix 1M 3-D points, sigma 0.05, r 0.2, brute-force evolve iteration",
+                   "points": n, "dim": d, "r": r, "chosen": K, "samples_per_lane": args.ppt},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "evolve_scan_kernel", "avg_launch_ms": scan_ms,
+                     "flops_per_launch": flops},
+        "secondary": secondary,
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(P, S0, r)
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -81,7 +81,17 @@ typedef struct rdhip_timers {
     double refine_ms;
     int64_t evolve_iterations;
     int64_t rounds;
+    double scan_kernel_ms;   /* HIP-event time of the evolve scan kernel (RDHIP_OPT_PROFILE) */
+    int64_t scan_launches;
+    double iteration_ms;     /* HIP-event time of whole evolve iterations (RDHIP_OPT_PROFILE) */
 } rdhip_timers;
+
+/* Tuning / instrumentation options (rdhip_set_option). */
+enum rdhip_option {
+    RDHIP_OPT_PPT = 0,             /* samples per lane of the evolve scan: 1, 2 or 4 (default 4) */
+    RDHIP_OPT_ITERS_PER_SYNC = 1,  /* evolve iterations queued between convergence checks (8) */
+    RDHIP_OPT_PROFILE = 2          /* 1: time the scan kernel and iterations with HIP events */
+};
 
 int rdhip_version(void);
 const char* rdhip_last_error(void);
@@ -107,6 +117,9 @@ int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double
                                 int64_t max_tile_capacity, const int64_t* tiles_per_dim,
                                 int refinement, int64_t max_evolve_iters, int64_t* out_count);
 
+/* Sets a tuning / instrumentation option (enum rdhip_option). */
+int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value);
+
 /* Results of the last run (or phase). */
 int rdhip_get_count(rdhip_ctx* ctx, int64_t* out_count);
 int rdhip_get_points(rdhip_ctx* ctx, void* out, int layout, int64_t stride, int on_device);
@@ -123,6 +136,10 @@ int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layo
 
 /* Evolve the chosen set with radius r until convergence (or max_iters > 0 iterations). */
 int rdhip_evolve(rdhip_ctx* ctx, double r, int64_t max_iters, int64_t* out_iters);
+
+/* Exactly `iters` evolve iterations with radius r, ignoring convergence (a converged set
+ * stays where it is).  The benchmark's unit of work and the forward pass of the evolve VJP. */
+int rdhip_evolve_fixed(rdhip_ctx* ctx, double r, int64_t iters);
 
 /* One evolve iteration.  Optional host outputs: labels[n] (chosen index of every point, or
  * -1 outside every r-ball), counts[K], sums[K*dim] (ROW_MAJOR, value type of the context),

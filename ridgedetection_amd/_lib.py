@@ -29,14 +29,16 @@ EXPORTS = [
     "rdhip_version", "rdhip_last_error", "rdhip_create", "rdhip_destroy", "rdhip_set_points",
     "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
-    "rdhip_decimate",
+    "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed",
 ]
+OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE = 0, 1, 2
 
 
 class Timers(C.Structure):
     _fields_ = [("whole_ms", C.c_double), ("build_ms", C.c_double), ("choose_ms", C.c_double),
                 ("evolve_ms", C.c_double), ("decimate_ms", C.c_double), ("refine_ms", C.c_double),
-                ("evolve_iterations", C.c_int64), ("rounds", C.c_int64)]
+                ("evolve_iterations", C.c_int64), ("rounds", C.c_int64),
+                ("scan_kernel_ms", C.c_double), ("scan_launches", C.c_int64), ("iteration_ms", C.c_double)]
 
 
 _lib = None
@@ -67,6 +69,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_evolve": (i32, [vp, C.c_double, i64, C.POINTER(i64)]),
         "rdhip_evolve_iteration": (i32, [vp, C.c_double, vp, vp, vp, vp, C.POINTER(i32)]),
         "rdhip_decimate": (i32, [vp, C.c_double, i32, vp, C.POINTER(i64)]),
+        "rdhip_set_option": (i32, [vp, i32, i64]),
+        "rdhip_evolve_fixed": (i32, [vp, C.c_double, i64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -176,6 +180,12 @@ class Context:
         it = C.c_int64()
         self._check(self.lib.rdhip_evolve(self.h, float(r), int(max_iters), C.byref(it)))
         return self.points(), it.value
+
+    def set_option(self, option, value):
+        self._check(self.lib.rdhip_set_option(self.h, int(option), int(value)))
+
+    def evolve_fixed(self, r, iters):
+        self._check(self.lib.rdhip_evolve_fixed(self.h, float(r), int(iters)))
 
     def evolve_iteration(self, r):
         K = self.count()

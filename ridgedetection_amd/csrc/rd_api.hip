@@ -203,6 +203,43 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->refine_ms = e.timers.refineMs;
         out->evolve_iterations = e.timers.evolveIterations;
         out->rounds = e.timers.rounds;
+        out->scan_kernel_ms = e.timers.scanKernelMs;
+        out->scan_launches = e.timers.scanLaunches;
+        out->iteration_ms = e.timers.iterationMs;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
+    return withEngine(ctx, [&](auto& e, auto) -> int {
+        switch (option) {
+            case RDHIP_OPT_PPT:
+                if (value != 1 && value != 2 && value != 4) return fail(RDHIP_ERR_INVALID, "ppt must be 1, 2 or 4");
+                if (e.options.ppt != (int)value) {
+                    e.options.ppt = (int)value;
+                    e.rebuildWorkItems();
+                }
+                return RDHIP_OK;
+            case RDHIP_OPT_ITERS_PER_SYNC:
+                if (value < 1 || value > 1024) return fail(RDHIP_ERR_INVALID, "iterations per sync must be in [1, 1024]");
+                e.options.itersPerSync = (int)value;
+                return RDHIP_OK;
+            case RDHIP_OPT_PROFILE:
+                e.options.profile = value != 0;
+                return RDHIP_OK;
+        }
+        return fail(RDHIP_ERR_INVALID, "unknown option");
+    });
+}
+
+int rdhip_evolve_fixed(rdhip_ctx* ctx, double r, int64_t iters) {
+    if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    if (iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration count");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points (call rdhip_choose first)");
+        e.setRadii({(T)r}, {(T)r});
+        e.evolveFixed(iters);
         return RDHIP_OK;
     });
 }

@@ -41,8 +41,9 @@ __global__ __launch_bounds__(kThreads) void choose_filter_kernel(ChooseArgs<T> a
     const T* __restrict__ Sb = a.S + sd.sOff;
     for (int i = it.start + threadIdx.x; i < it.end; i += kThreads) {
         T p[DIM];
+        const int64_t src = point_of(a.pIndex, sd.pOff + i);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) p[d] = a.P[d * a.pStride + sd.pOff + i];
+        for (int d = 0; d < DIM; ++d) p[d] = a.P[d * a.pStride + src];
         a.covered[sd.pOff + i] = covered_by<T, DIM>(p, Sb, a.sStride, 0, K, rSqr) ? 1 : 0;
     }
 }
@@ -61,9 +62,9 @@ __global__ __launch_bounds__(kThreads) void choose_block_kernel(ChooseArgs<T> a,
     const T rSqr = a.segRSqr[seg];
     const int64_t ss = a.sStride;
     T* __restrict__ Sb = a.S + sd.sOff;
-    const T* __restrict__ Pb = a.P + sd.pOff;
     const int64_t ps = a.pStride;
     int* __restrict__ cidx = a.chosenIdx + sd.sOff;
+    auto coord = [&](int i, int d) { return a.P[d * ps + point_of(a.pIndex, sd.pOff + i)]; };
 
     int from, to;
     if (chunk < 0) {
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void choose_block_kernel(ChooseArgs<T> a,
     int K;
     if (from == 0) {
         // sample 0 is always chosen
-        if (threadIdx.x < DIM) Sb[threadIdx.x * ss] = Pb[threadIdx.x * ps];
+        if (threadIdx.x < DIM) Sb[threadIdx.x * ss] = coord(0, threadIdx.x);
         if (threadIdx.x == 0) cidx[0] = 0;
         K = 1;
         from = 1;
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(kThreads) void choose_block_kernel(ChooseArgs<T> a,
         T p[DIM];
         bool live = i < to;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) p[d] = live ? Pb[d * ps + i] : T(0);
+        for (int d = 0; d < DIM; ++d) p[d] = live ? coord(i, d) : T(0);
         if (live) {
             if (useFlags && a.covered[sd.pOff + i]) live = false;
             else if (covered_by<T, DIM>(p, Sb, ss, useFlags ? kChunkStart : 0, K, rSqr)) live = false;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void choose_block_kernel(ChooseArgs<T> a,
                 T pq[DIM], s[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    pq[d] = Pb[d * ps + c0 + t];
+                    pq[d] = coord(c0 + t, d);
                     s[d] = newS[threadIdx.x][d];
                 }
                 cov = exact_sqdist<T, DIM>(pq, s) <= rSqr;

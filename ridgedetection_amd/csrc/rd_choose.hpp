@@ -9,6 +9,7 @@ template <typename T>
 struct ChooseArgs {
     const T* P;
     int64_t pStride;
+    const int* pIndex;        // optional position -> point map
     T* S;
     int64_t sStride;
     const SegDesc* segs;

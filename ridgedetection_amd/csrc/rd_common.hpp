@@ -59,6 +59,12 @@ struct WorkItem {
     int32_t pad;
 };
 
+// Segments address their samples by position; with an index list (tiles, whose neighbour
+// samples repeat other tiles' points) position p holds point pIndex[p] of the SoA buffer.
+__device__ __forceinline__ int64_t point_of(const int* __restrict__ pIndex, int64_t pos) {
+    return pIndex ? (int64_t)pIndex[pos] : pos;
+}
+
 // ---------------------------------------------------------------------------------
 // wave64 helpers
 // ---------------------------------------------------------------------------------

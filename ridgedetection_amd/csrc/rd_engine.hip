@@ -68,6 +68,7 @@ Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
 
 template <typename T>
 Engine<T>::~Engine() {
+    for (auto ev : events_) (void)hipEventDestroy(ev);
     if (decWs_.nbr) (void)hipFree(decWs_.nbr);
     for (auto* b : chooseItems_) delete b;
 }
@@ -110,13 +111,14 @@ void Engine<T>::setSingleSegment(T r1, T r2) {
     s.nP = (int32_t)nPoints_;
     s.nOwn = (int32_t)nPoints_;
     s.group = 0;
-    setSegments({s}, {std::max<int64_t>(nPoints_, 1)}, {r1}, {r2});
+    setSegments({s}, {std::max<int64_t>(nPoints_, 1)}, {r1}, {r2}, false);
 }
 
 template <typename T>
 void Engine<T>::setSegments(const std::vector<SegDesc>& segs, const std::vector<int64_t>& capacities,
-                            const std::vector<T>& r1, const std::vector<T>& r2) {
+                            const std::vector<T>& r1, const std::vector<T>& r2, bool indexed) {
     const int ns = (int)segs.size();
+    useIndex_ = indexed;
     if (capacities.size() != segs.size() || r1.size() != segs.size() || r2.size() != segs.size())
         throw std::invalid_argument("segment arrays differ in length");
     segs_ = segs;
@@ -230,6 +232,7 @@ void Engine<T>::ensureEvolveScratch() {
     moved_.reserve(slots);
     sums_.reserve((size_t)sStride_ * dim_);
     sortWs_.reserve(sort_workspace_bytes(totalP, (int)std::max<int64_t>(totalSlots_, 2)));
+    sortedP_.reserve((size_t)totalP * dim_);
     check(hipMemcpyAsync(dN_.get(), &totalP, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
     // decimate scratch: elemBase, groupOff/Cnt/Left, elemSlot, elemGroup, c1, c2, rowOff, alive
     const int ns = std::max<int>((int)segs_.size(), 1);
@@ -242,6 +245,15 @@ template <typename T>
 void Engine<T>::fillSentinels() {
     launch_fill_sentinel<T>(S_.get(), sStride_, dim_, dSegs_.get(), dSegK_.get(), (int)segs_.size(), st_);
     check(hipGetLastError(), "fill_sentinel");
+}
+
+template <typename T>
+void Engine<T>::updateKeyBound() {
+    std::vector<int> K = countsHost();
+    int64_t kb = 0;
+    for (size_t s = 0; s < K.size(); ++s)
+        if (K[s] > 0) kb = std::max<int64_t>(kb, segs_[s].sOff + K[s]);
+    keyBound_ = kb;
 }
 
 // ------------------------------------------------------------------------------------
@@ -266,7 +278,7 @@ void Engine<T>::choose() {
     check(hipMemcpyAsync(dLarge.get(), large.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "large");
     int zero = 0;
     check(hipMemcpyAsync(flags_.get() + 2, &zero, sizeof(int), hipMemcpyHostToDevice, st_), "overflow");
-    ChooseArgs<T> a{P_.get(), pStride_, S_.get(), sStride_, dSegs_.get(), ns, dSegK_.get(), dSegR1Sqr_.get(),
+    ChooseArgs<T> a{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), ns, dSegK_.get(), dSegR1Sqr_.get(),
                     covered_.get(), chosenIdx_.get(), flags_.get() + 2, dLarge.get()};
     // small segments: one block each, whole segment
     check(launch_choose_block<T>(a, dim_, -1, 0, st_), "choose_block");
@@ -297,6 +309,7 @@ void Engine<T>::choose() {
     check(hipMemcpyAsync(&ovf, flags_.get() + 2, sizeof(int), hipMemcpyDeviceToHost, st_), "overflow");
     check(hipStreamSynchronize(st_), "choose");
     if (ovf) throw std::runtime_error("choose: chosen points exceed a segment's capacity");
+    updateKeyBound();
     timers.chooseMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -304,25 +317,70 @@ void Engine<T>::choose() {
 // evolve
 // ------------------------------------------------------------------------------------
 template <typename T>
-void Engine<T>::enqueueIteration(bool record) {
-    ScanArgs<T> sa{P_.get(), pStride_, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+void Engine<T>::enqueueIteration(bool record, int slot) {
+    ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                    dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get()};
+    const bool ev = options.profile && slot >= 0;
+    if (ev) {
+        while ((int)events_.size() < 3 * (slot + 1)) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            events_.push_back(e);
+        }
+        check(hipEventRecord(events_[3 * slot], st_), "event");
+    }
     check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, st_), "evolve_scan");
+    if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
     const int64_t totalP = totalP_;
-    check(stable_sort_by_key(labels_.get(), nullptr, dN_.get(), totalP, (int)std::max<int64_t>(totalSlots_, 2),
+    // labels are live slots < keyBound_ (counts only shrink between refreshes)
+    const int64_t kb = std::max<int64_t>(keyBound_, 1);
+    check(stable_sort_by_key(labels_.get(), nullptr, dN_.get(), totalP, (int)std::max<int64_t>(kb, 2),
                              keys_.get(), members_.get(), dM_.get(), sortWs_.get(), st_, flags_.get()),
           "sort");
-    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, chainOff_.get(), totalSlots_,
-                       0, flags_.get());
-    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, chainEnd_.get(), totalSlots_,
-                       0, flags_.get());
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), kb, 0, flags_.get());
+    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainEnd_.get(), kb, 0, flags_.get());
     check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), flags_.get(), st_),
           "chain_bounds");
-    ShiftArgs<T> ha{P_.get(), pStride_, S_.get(), sStride_, totalSlots_, dSlotSeg_.get(), dSegs_.get(),
-                    dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), chainOff_.get(),
-                    chainEnd_.get(), flags_.get(), record ? counts_.get() : nullptr,
+    ShiftArgs<T> ha{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
+                    dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
+                    totalP, chainOff_.get(), chainEnd_.get(), flags_.get(), record ? counts_.get() : nullptr,
                     record ? sums_.get() : nullptr, record ? moved_.get() : nullptr};
-    check(launch_chain_sum_shift<T>(ha, dim_, st_), "chain_sum_shift");
+    check(launch_chain_sum_shift<T>(ha, dim_, totalP, st_), "chain_sum_shift");
+    if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
+}
+
+template <typename T>
+void Engine<T>::readEvents(int used) {
+    for (int b = 0; b < used; ++b) {
+        float a = 0, w = 0;
+        check(hipEventElapsedTime(&a, events_[3 * b], events_[3 * b + 1]), "elapsed");
+        check(hipEventElapsedTime(&w, events_[3 * b], events_[3 * b + 2]), "elapsed");
+        timers.scanKernelMs += a;
+        timers.iterationMs += w;
+        timers.scanLaunches += 1;
+    }
+}
+
+template <typename T>
+void Engine<T>::evolveFixed(long long iters) {
+    const int ns = (int)segs_.size();
+    if (ns == 0 || iters <= 0) return;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> zeros(ns, 0);
+    int f[2] = {0, 0};
+    check(hipMemcpyAsync(dSegDone_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
+    check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    const int batch = std::max(1, options.itersPerSync);
+    long long done = 0;
+    while (done < iters) {
+        const int nb = (int)std::min<long long>(batch, iters - done);
+        for (int b = 0; b < nb; ++b) enqueueIteration(false, b);
+        check(hipStreamSynchronize(st_), "evolveFixed");
+        if (options.profile) readEvents(nb);
+        done += nb;
+    }
+    timers.evolveIterations += iters;
+    timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 template <typename T>
@@ -347,13 +405,14 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     for (;;) {
         const int batch = std::max(1, options.itersPerSync);
         for (int b = 0; b < batch; ++b) {
-            enqueueIteration(false);
+            enqueueIteration(false, b);
             check(launch_evolve_finalize(ns, dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
                                          flags_.get(), flags_.get() + 1, st_),
                   "finalize");
         }
         check(hipMemcpyAsync(f, flags_.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "flags");
         check(hipStreamSynchronize(st_), "evolve");
+        if (options.profile) readEvents(batch);
         iters = f[1];
         if (f[0]) break;
         if (iters >= kEvolveGuard)
@@ -450,6 +509,7 @@ void Engine<T>::decimate(bool noCopyRule, std::vector<unsigned char>* keep) {
         check(hipMemcpyAsync(keep->data(), keepBuf.get(), totalSlots_, hipMemcpyDeviceToHost, st_), "keep");
     }
     check(hipStreamSynchronize(st_), "decimate");
+    updateKeyBound();
     timers.decimateMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -598,6 +658,7 @@ void Engine<T>::setChosen(int seg, const T* S, int64_t K, int layout) {
               "idx");
     fillSentinels();
     check(hipStreamSynchronize(st_), "setChosen");
+    updateKeyBound();
 }
 
 template class Engine<float>;

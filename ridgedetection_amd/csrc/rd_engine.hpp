@@ -55,6 +55,9 @@ enum Layout { LAYOUT_ROW_MAJOR = 0, LAYOUT_COL_MAJOR = 1, LAYOUT_SOA = 2 };
 struct Timers {
     double wholeMs = 0, buildMs = 0, chooseMs = 0, evolveMs = 0, decimateMs = 0, refineMs = 0;
     long long evolveIterations = 0, rounds = 0;
+    double scanKernelMs = 0;      // HIP-event time of the evolve scan kernel (when profiling)
+    long long scanLaunches = 0;
+    double iterationMs = 0;       // HIP-event time of whole evolve iterations (when profiling)
 };
 
 struct RunOptions {
@@ -62,6 +65,7 @@ struct RunOptions {
     long long maxRounds = 0;        // 0 = until the count is stable (reference)
     int ppt = 4;                    // samples per lane in the evolve scan
     int itersPerSync = 8;           // evolve iterations enqueued between convergence checks
+    bool profile = false;           // record HIP events around the scan kernel / iterations
 };
 
 template <typename T>
@@ -85,18 +89,28 @@ public:
     // Defines the segments over the SoA buffer (pOff/nP/nOwn/group must be set; sOff and
     // sCap are assigned here from capacities).  Group g = consecutive segments with
     // group == g.  r1/r2 are per segment.
+    // indexed: segment positions map to points through the index list filled via
+    // reserveIndex() (tiles whose neighbour samples repeat other tiles' points).
     void setSegments(const std::vector<SegDesc>& segs, const std::vector<int64_t>& capacities,
-                     const std::vector<T>& r1, const std::vector<T>& r2);
+                     const std::vector<T>& r1, const std::vector<T>& r2, bool indexed = false);
+    int* reserveIndex(size_t positions) { return pIndexBuf_.reserve(positions); }
     void setSingleSegment(T r1, T r2);
     // Replaces the radii of the current segments without touching their chosen points.
     void setRadii(const std::vector<T>& r1, const std::vector<T>& r2);
     int numSegments() const { return (int)segs_.size(); }
+    void rebuildWorkItems() {
+        buildWorkItems();
+        check(hipStreamSynchronize(st_), "rebuildWorkItems");
+    }
 
     // -- phases --------------------------------------------------------------------
     void choose();
     // Runs evolve iterations on every segment whose flag in `active` is set (all when
     // empty) until each converges or reaches maxIter.  Returns the number of iterations.
     long long evolve(long long maxIter, const std::vector<char>& active = {});
+    // Exactly `iters` evolve iterations on every segment, ignoring convergence (benchmarks,
+    // frozen-membership forward passes).
+    void evolveFixed(long long iters);
     // One evolve iteration on all segments, recording labels/counts/sums/moved on host.
     int evolveIteration(std::vector<int>* labels, std::vector<int>* counts, std::vector<T>* sums,
                         std::vector<unsigned short>* moved);
@@ -128,6 +142,9 @@ private:
     // points
     DevBuf<T> P_;
     int64_t nPoints_ = 0, pStride_ = 0;
+    DevBuf<int> pIndexBuf_;
+    bool useIndex_ = false;
+    const int* pIndexPtr() const { return useIndex_ ? pIndexBuf_.get() : nullptr; }
     // segments
     std::vector<SegDesc> segs_;
     std::vector<int> groupBegin_, groupEnd_;
@@ -142,7 +159,9 @@ private:
     DevBuf<int> chosenIdx_;
     // evolve scratch
     DevBuf<int> labels_, keys_, members_, chainOff_, chainEnd_, counts_;
-    DevBuf<T> sums_;
+    DevBuf<T> sums_, sortedP_;
+    int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
+    void updateKeyBound();
     DevBuf<unsigned short> moved_;
     DevBuf<unsigned char> sortWs_;
     DevBuf<int64_t> dN_, dM_;
@@ -158,7 +177,9 @@ private:
 
     void buildWorkItems();
     void ensureEvolveScratch();
-    void enqueueIteration(bool record);
+    void enqueueIteration(bool record, int slot = -1);
+    void readEvents(int used);
+    std::vector<hipEvent_t> events_;   // per batch slot: scan start, scan end, iteration end
     void fillSentinels();
 };
 

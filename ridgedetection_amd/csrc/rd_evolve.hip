@@ -20,9 +20,11 @@
 //   2. stable sort of the sample indices by label (rd_sort.hip): the members of every
 //      chosen point in sample order.
 //   3. chain_bounds -- start/end of every chosen point's member run.
-//   4. chain_sum_shift -- one lane per chosen point adds its members in sample order
-//      (the reference's serial summation, so sums are bit-identical), then applies the
-//      shift rule and raises the segment's "moved" flag.
+//   4. gather_members + chain_sum_shift -- the members' coordinates are copied into chain
+//      order; then one wave per chosen point reads 64 members per step (coalesced) and
+//      folds them into the running sum lane by lane, i.e. in sample order (the reference's
+//      serial summation, so sums are bit-identical), applies the shift rule and raises
+//      the segment's "moved" flag.
 //   5. evolve_finalize -- per-segment iteration count and convergence; global all-done.
 #include "rd_evolve.hpp"
 
@@ -65,7 +67,6 @@ __global__ __launch_bounds__(kScanThreads) void evolve_scan_kernel(ScanArgs<T> a
     }
     const int K = a.segK[it.seg];
     const T rSqr = a.segRSqr[it.seg];
-    const T* __restrict__ Pb = a.P + sd.pOff;
     const T* __restrict__ Sb = a.S + sd.sOff;
     const int64_t ps = a.pStride;
     const int64_t ss = a.sStride;
@@ -77,8 +78,9 @@ __global__ __launch_bounds__(kScanThreads) void evolve_scan_kernel(ScanArgs<T> a
         int i = it.start + tid + j * kScanThreads;
         idx[j] = i;
         bool v = i < it.end;
+        const int64_t src = v ? point_of(a.pIndex, sd.pOff + i) : 0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) p[j][d] = v ? Pb[d * ps + i] : T(0);
+        for (int d = 0; d < DIM; ++d) p[j][d] = v ? a.P[d * ps + src] : T(0);
     }
     if (K <= 0) {
 #pragma unroll
@@ -199,46 +201,80 @@ __global__ __launch_bounds__(256) void chain_bounds_kernel(const int* __restrict
 }
 
 // ---------------------------------------------------------------------------------
-// 4. ordered sums + shift toward the mass centre
+// 4a. member coordinates in chain order (contiguous per chosen point)
+// ---------------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void gather_members_kernel(const T* __restrict__ P, int64_t pStride,
+                                                             const int* __restrict__ pIndex,
+                                                             const int* __restrict__ members,
+                                                             const int64_t* __restrict__ mPtr,
+                                                             T* __restrict__ sorted, int64_t mStride,
+                                                             const int* __restrict__ allDone) {
+    if (*allDone) return;
+    const int64_t M = *mPtr;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < M; i += (int64_t)gridDim.x * 256) {
+        const int64_t n = point_of(pIndex, members[i]);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sorted[d * mStride + i] = P[d * pStride + n];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int lane);
+template <>
+__device__ __forceinline__ float lane_value<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ __forceinline__ double lane_value<double>(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// ---------------------------------------------------------------------------------
+// 4b. ordered sums + shift toward the mass centre: one wave per chosen point.  The wave
+// loads 64 consecutive members per step (coalesced) and folds them into the running sum
+// in member order, lane 0 first, exactly the reference's serial summation.
 // ---------------------------------------------------------------------------------
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
     if (*a.allDone) return;
-    const int64_t slot = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t slot = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
     if (slot >= a.totalSlots) return;
     const int seg = a.slotSeg[slot];
     if (seg < 0 || a.segDone[seg]) return;
     const SegDesc sd = a.segs[seg];
     const int k = (int)(slot - sd.sOff);
     if (k >= a.segK[seg]) return;
+    const int lane = lane_id();
     const int off = a.chainOff[slot];
     const int cnt = a.chainEnd[slot] - off;
     T sum[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) sum[d] = T(0);
-    const int* __restrict__ mem = a.members + off;
-    int i = 0;
-    // members in sample order; loads are issued ahead of the dependent adds
-    for (; i + 4 <= cnt; i += 4) {
-        int n0 = mem[i], n1 = mem[i + 1], n2 = mem[i + 2], n3 = mem[i + 3];
-        T x[4][DIM];
+    const T* __restrict__ src = a.sortedP + off;
+    int c0 = 0;
+    for (; c0 + kWave <= cnt; c0 += kWave) {
+        T v[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            x[0][d] = a.P[d * a.pStride + n0];
-            x[1][d] = a.P[d * a.pStride + n1];
-            x[2][d] = a.P[d * a.pStride + n2];
-            x[3][d] = a.P[d * a.pStride + n3];
-        }
+        for (int d = 0; d < DIM; ++d) v[d] = src[d * a.mStride + c0 + lane];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < kWave; ++q)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + x[q][d];
+            for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + lane_value<T>(v[d], q);
     }
-    for (; i < cnt; ++i) {
-        int n0 = mem[i];
+    if (c0 < cnt) {
+        const int m = cnt - c0;
+        T v[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + a.P[d * a.pStride + n0];
+        for (int d = 0; d < DIM; ++d) v[d] = lane < m ? src[d * a.mStride + c0 + lane] : T(0);
+        for (int q = 0; q < m; ++q)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + lane_value<T>(v[d], q);
     }
+    if (lane != 0) return;
     if (a.outCounts) a.outCounts[slot] = cnt;
     if (a.outSums)
 #pragma unroll
@@ -332,11 +368,14 @@ hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int p
 }
 
 template <typename T>
-hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, hipStream_t st) {
-    const int64_t blocks = (a.totalSlots + 255) / 256;
+hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, hipStream_t st) {
+    const int64_t blocks = (a.totalSlots + (256 / kWave) - 1) / (256 / kWave);
     if (blocks <= 0) return hipSuccess;
+    const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
+        hipLaunchKernelGGL((gather_members_kernel<T, DIM>), dim3(gblocks), dim3(256), 0, st, a.P, a.pStride,
+                           a.pIndex, a.members, a.mPtr, a.sortedP, a.mStride, a.allDone);
         hipLaunchKernelGGL((chain_sum_shift_kernel<T, DIM>), dim3((unsigned)blocks), dim3(256), 0, st, a);
         ok = true;
     });
@@ -370,8 +409,8 @@ hipError_t launch_fill_sentinel(T* S, int64_t sStride, int dim, const SegDesc* s
 
 template hipError_t launch_evolve_scan<float>(const ScanArgs<float>&, int, int, int, hipStream_t);
 template hipError_t launch_evolve_scan<double>(const ScanArgs<double>&, int, int, int, hipStream_t);
-template hipError_t launch_chain_sum_shift<float>(const ShiftArgs<float>&, int, hipStream_t);
-template hipError_t launch_chain_sum_shift<double>(const ShiftArgs<double>&, int, hipStream_t);
+template hipError_t launch_chain_sum_shift<float>(const ShiftArgs<float>&, int, int64_t, hipStream_t);
+template hipError_t launch_chain_sum_shift<double>(const ShiftArgs<double>&, int, int64_t, hipStream_t);
 template hipError_t launch_fill_sentinel<float>(float*, int64_t, int, const SegDesc*, const int*, int, hipStream_t);
 template hipError_t launch_fill_sentinel<double>(double*, int64_t, int, const SegDesc*, const int*, int, hipStream_t);
 

@@ -13,6 +13,7 @@ template <typename T>
 struct ScanArgs {
     const T* P;
     int64_t pStride;
+    const int* pIndex;      // optional position -> point map
     const T* S;
     int64_t sStride;
     const WorkItem* items;
@@ -28,6 +29,7 @@ template <typename T>
 struct ShiftArgs {
     const T* P;
     int64_t pStride;
+    const int* pIndex;      // optional position -> point map
     T* S;
     int64_t sStride;
     int64_t totalSlots;
@@ -37,6 +39,9 @@ struct ShiftArgs {
     const int* segDone;
     int* segMoved;
     const int* members;     // sample indices sorted by (label, sample)
+    const int64_t* mPtr;    // number of members (device)
+    T* sortedP;             // SoA member coordinates in chain order (scratch)
+    int64_t mStride;
     const int* chainOff;
     const int* chainEnd;
     const int* allDone;
@@ -47,8 +52,10 @@ struct ShiftArgs {
 
 template <typename T>
 hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, hipStream_t st);
+// Gathers the members' coordinates in chain order, then sums and shifts; slots
+// [0, a.totalSlots) are visited (pass the live bound, not the capacity).
 template <typename T>
-hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, hipStream_t st);
+hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, hipStream_t st);
 hipError_t launch_chain_bounds(const int* keys, const int64_t* dM, int64_t maxM, int* chainOff,
                                int* chainEnd, const int* allDone, hipStream_t st);
 hipError_t launch_evolve_finalize(int numSegs, int* segDone, int* segMoved, int* segIter, int maxIter,

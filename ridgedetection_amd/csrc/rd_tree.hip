@@ -1,0 +1,674 @@
+// rd_tree.hip -- level-synchronous construction of the tile trees on the device.
+//
+// Every level handles all the tiles being split at once:
+//   1. bin      -- the child of every own sample (histogram map for LOCAL, half-open cell
+//                  search for GROUPED / EXTENDED), then a stable sort by child id;
+//   2. nearby   -- EXTENDED only: (child cell, sample) pairs for every sample of
+//                  (parent own, parent neighbours) that is near the cell, counted, emitted
+//                  and stably sorted by child id;
+//   3. bounds   -- LOCAL only: tight bounds of every child's own samples;
+// and the host keeps the skeleton (counts, bounds, leaf/split decisions).  Cell bounds are
+// computed on the host with the reference's expressions (min + c * step, last cell ending at
+// the parent max) and uploaded as per-dimension interval tables, so device and host agree
+// bit for bit.
+#include "rd_tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+namespace rdhip {
+
+namespace {
+
+template <typename T>
+struct ParentDev {
+    const int* idx;      // own samples: idx[off .. off+cnt)
+    int64_t off, cnt;
+    int64_t outOff;      // position of its own samples in the level's key array
+    const int* nbIdx;    // EXTENDED neighbour samples
+    int64_t nbOff, nbCnt;
+    int64_t posOff;      // position of (own, nb) in the level's nearby-count array
+    int childBase;
+    int ivOff;           // first interval of dimension 0 in the interval tables
+    int nbins[kMaxDim];
+    int stride[kMaxDim];
+    T mn[kMaxDim];
+    T dist[kMaxDim];
+};
+
+struct RangeCopy {
+    const int* src;
+    int64_t srcOff, cnt, dstOff;
+};
+
+template <typename T>
+__device__ int bin_local(const T* x, const ParentDev<T>& pd, int dim) {
+    const T eps = FpTraits<T>::eps;
+    int64_t id = 0, total = 1;
+    for (int d = 0; d < dim; ++d) {
+        const int n = pd.nbins[d];
+        int64_t b = 0;
+        if (!(pd.dist[d] < eps)) {
+            const T normCord = fabs(x[d] - pd.mn[d]);
+            const T step = pd.dist[d] / T(n);
+            if (fabs(normCord - pd.dist[d]) <= eps || normCord - pd.dist[d] > 0)
+                b = n - 1;
+            else
+                b = (int64_t)floor(normCord / step);
+        }
+        id += b * pd.stride[d];
+        total *= n;
+    }
+    return (int)(id < total ? id : total - 1);
+}
+
+// index of the half-open interval [lo, hi) holding v, or -1
+template <typename T>
+__device__ int find_interval(const T* lo, const T* hi, int n, T v) {
+    // last c with lo[c] <= v (lo is non-decreasing)
+    int a = 0, b = n - 1, c = -1;
+    while (a <= b) {
+        const int m = (a + b) >> 1;
+        if (lo[m] <= v) {
+            c = m;
+            a = m + 1;
+        } else {
+            b = m - 1;
+        }
+    }
+    if (c >= 0 && v < hi[c]) return c;
+    return -1;
+}
+
+template <typename T>
+__device__ int bin_cell(const T* x, const ParentDev<T>& pd, const T* ivLo, const T* ivHi, int dim) {
+    int id = 0, o = pd.ivOff;
+    for (int d = 0; d < dim; ++d) {
+        const int c = find_interval(ivLo + o, ivHi + o, pd.nbins[d], x[d]);
+        if (c < 0) return -1;
+        id += c * pd.stride[d];
+        o += pd.nbins[d];
+    }
+    return id;
+}
+
+// per-dimension range of cells whose r-expanded interval holds v: [c0, c1]
+template <typename T>
+__device__ void near_range(const T* lo, const T* hi, int n, T v, T r, int& c0, int& c1) {
+    // first c with !(v >= hi[c] + r)  (hi non-decreasing)
+    int a = 0, b = n - 1, f = n;
+    while (a <= b) {
+        const int m = (a + b) >> 1;
+        if (!(v >= hi[m] + r)) {
+            f = m;
+            b = m - 1;
+        } else {
+            a = m + 1;
+        }
+    }
+    // last c with !(v < lo[c] - r)
+    a = 0;
+    b = n - 1;
+    int l = -1;
+    while (a <= b) {
+        const int m = (a + b) >> 1;
+        if (!(v < lo[m] - r)) {
+            l = m;
+            a = m + 1;
+        } else {
+            b = m - 1;
+        }
+    }
+    c0 = f;
+    c1 = l;
+}
+
+template <typename T, bool COUNT>
+__device__ int near_cells(const T* x, const ParentDev<T>& pd, const T* ivLo, const T* ivHi, int dim, T r,
+                          int* outKeys) {
+    int c0[kMaxDim], c1[kMaxDim], c[kMaxDim];
+    int o = pd.ivOff;
+    int inside = 1;
+    int own[kMaxDim];
+    for (int d = 0; d < dim; ++d) {
+        near_range(ivLo + o, ivHi + o, pd.nbins[d], x[d], r, c0[d], c1[d]);
+        if (c0[d] > c1[d]) return 0;
+        own[d] = find_interval(ivLo + o, ivHi + o, pd.nbins[d], x[d]);
+        inside &= own[d] >= 0;
+        c[d] = c0[d];
+        o += pd.nbins[d];
+    }
+    int emitted = 0;
+    for (;;) {
+        bool isOwn = inside;
+        int id = 0;
+        for (int d = 0; d < dim; ++d) {
+            isOwn = isOwn && c[d] == own[d];
+            id += c[d] * pd.stride[d];
+        }
+        if (!isOwn) {
+            if (!COUNT) outKeys[emitted] = pd.childBase + id;
+            ++emitted;
+        }
+        int d = 0;
+        for (; d < dim; ++d) {
+            if (++c[d] <= c1[d]) break;
+            c[d] = c0[d];
+        }
+        if (d == dim) break;
+    }
+    return emitted;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void iota_kernel(int* __restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = (int)i;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bin_kernel(const T* __restrict__ P, int64_t ps, int dim, int local,
+                                                  const ParentDev<T>* __restrict__ parents,
+                                                  const WorkItem* __restrict__ items, const T* __restrict__ ivLo,
+                                                  const T* __restrict__ ivHi, int* __restrict__ keys,
+                                                  int* __restrict__ vals) {
+    const WorkItem it = items[blockIdx.x];
+    const ParentDev<T>& pd = parents[it.seg];
+    for (int i = it.start + threadIdx.x; i < it.end; i += 256) {
+        const int p = pd.idx[pd.off + i];
+        T x[kMaxDim];
+        for (int d = 0; d < dim; ++d) x[d] = P[d * ps + p];
+        const int b = local ? bin_local(x, pd, dim) : bin_cell(x, pd, ivLo, ivHi, dim);
+        keys[pd.outOff + i] = b >= 0 ? pd.childBase + b : -1;
+        vals[pd.outOff + i] = p;
+    }
+}
+
+// EXTENDED: positions of a parent are its own samples then its neighbour samples
+template <typename T, bool COUNT>
+__global__ __launch_bounds__(256) void nearby_kernel(const T* __restrict__ P, int64_t ps, int dim, T r,
+                                                     const ParentDev<T>* __restrict__ parents,
+                                                     const WorkItem* __restrict__ items, const T* __restrict__ ivLo,
+                                                     const T* __restrict__ ivHi, int* __restrict__ counts,
+                                                     int* __restrict__ keys, int* __restrict__ vals) {
+    const WorkItem it = items[blockIdx.x];
+    const ParentDev<T>& pd = parents[it.seg];
+    for (int i = it.start + threadIdx.x; i < it.end; i += 256) {
+        const int p = i < pd.cnt ? pd.idx[pd.off + i] : pd.nbIdx[pd.nbOff + (i - pd.cnt)];
+        T x[kMaxDim];
+        for (int d = 0; d < dim; ++d) x[d] = P[d * ps + p];
+        const int64_t pos = pd.posOff + i;
+        if (COUNT) {
+            counts[pos] = near_cells<T, true>(x, pd, ivLo, ivHi, dim, r, nullptr);
+        } else {
+            const int o = counts[pos];
+            const int m = near_cells<T, false>(x, pd, ivLo, ivHi, dim, r, keys + o);
+            for (int q = 0; q < m; ++q) vals[o + q] = p;
+        }
+    }
+}
+
+// tight bounds of segment s = idx[off[s] .. end[s]) (min/max with the reference's comparisons)
+template <typename T>
+__global__ __launch_bounds__(256) void seg_bounds_kernel(const T* __restrict__ P, int64_t ps, int dim,
+                                                         const int* __restrict__ idx,
+                                                         const int* __restrict__ off, const int* __restrict__ end,
+                                                         const int* __restrict__ sel, T* __restrict__ out) {
+    __shared__ T smn[256], smx[256];
+    const int s = sel[blockIdx.x];
+    const int o = off[s], e = end[s];
+    for (int d = 0; d < dim; ++d) {
+        T mn = FpTraits<T>::big, mx = -FpTraits<T>::big;
+        for (int i = o + threadIdx.x; i < e; i += 256) {
+            const T v = P[d * ps + idx[i]];
+            mn = (v < mn) ? v : mn;
+            mx = (v > mx) ? v : mx;
+        }
+        smn[threadIdx.x] = mn;
+        smx[threadIdx.x] = mx;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) {
+                const T a = smn[threadIdx.x + h], b = smx[threadIdx.x + h];
+                smn[threadIdx.x] = (a < smn[threadIdx.x]) ? a : smn[threadIdx.x];
+                smx[threadIdx.x] = (b > smx[threadIdx.x]) ? b : smx[threadIdx.x];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[(int64_t)blockIdx.x * 2 * dim + 2 * d] = smn[0];
+            out[(int64_t)blockIdx.x * 2 * dim + 2 * d + 1] = smx[0];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_ranges_kernel(const RangeCopy* __restrict__ rc, int* __restrict__ dst) {
+    const RangeCopy r = rc[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < r.cnt; i += 256) dst[r.dstOff + i] = r.src[r.srcOff + i];
+}
+
+__global__ void zero_kernel(int* __restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = 0;
+}
+
+inline unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
+
+template <typename T>
+void calc_dist(const std::vector<T>& mn, const std::vector<T>& mx, std::vector<T>& dist) {
+    dist.resize(mn.size());
+    for (size_t k = 0; k < mn.size(); ++k) {
+        dist[k] = std::abs(mx[k] - mn[k]);
+        if (dist[k] <= std::numeric_limits<T>::epsilon()) dist[k] = 0;
+    }
+}
+
+template <typename T>
+size_t count_spheres(const std::vector<T>& dist, T r) {
+    size_t cnt = 1;
+    for (T dv : dist) {
+        size_t dc = static_cast<size_t>(std::ceil(dv / r));
+        cnt *= dc ? dc + 1 : 1;
+    }
+    return cnt;
+}
+
+template <typename T>
+bool boxes_overlap(const std::vector<T>& amn, const std::vector<T>& amx, const std::vector<T>& bmn,
+                   const std::vector<T>& bmx) {
+    for (size_t d = 0; d < amn.size(); ++d) {
+        bool res = (bmn[d] >= amn[d] && bmn[d] <= amx[d]) || (bmx[d] >= amn[d] && bmx[d] <= amx[d]) ||
+                   (amn[d] >= bmn[d] && amn[d] <= bmx[d]) || (amx[d] >= bmn[d] && amx[d] <= bmx[d]);
+        if (!res) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+template <typename T>
+TileTreeBuilder<T>::~TileTreeBuilder() = default;
+
+template <typename T>
+void TileTreeBuilder<T>::preorder(int t) {
+    if (tiles_[t].leaf)
+        leafTiles_.push_back(t);
+    else
+        for (int c : tiles_[t].children) preorder(c);
+}
+
+template <typename T>
+void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1, int64_t cap,
+                               const std::vector<int64_t>& tpd) {
+    kind_ = kind;
+    r1_ = r1;
+    cap_ = cap;
+    tiles_.clear();
+    rootChildren_.clear();
+    bufs_.clear();
+    leafTiles_.clear();
+    leaves_.clear();
+    const int dim = dim_;
+    const bool local = kind == TREE_LOCAL;
+    const bool ext = kind == TREE_EXTENDED;
+    if ((int)tpd.size() != dim) throw std::invalid_argument("tiles_per_dim needs one entry per dimension");
+
+    // root: identity list + global tight bounds
+    bufs_.emplace_back(new DevBuf<int>());
+    int* ident = bufs_[0]->reserve(std::max<int64_t>(n, 1));
+    hipLaunchKernelGGL((iota_kernel<T>), dim3(grid_of(n)), dim3(256), 0, st_, ident, n);
+    HTile root;
+    {
+        DevBuf<int> rOff, rEnd, rSel;
+        int z = 0, nn = (int)n;
+        rOff.reserve(1);
+        rEnd.reserve(1);
+        rSel.reserve(1);
+        check(hipMemcpyAsync(rOff.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
+        check(hipMemcpyAsync(rEnd.get(), &nn, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
+        check(hipMemcpyAsync(rSel.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
+        DevBuf<T> out;
+        out.reserve(2 * dim);
+        hipLaunchKernelGGL((seg_bounds_kernel<T>), dim3(1), dim3(256), 0, st_, P, ps, dim, ident, rOff.get(),
+                           rEnd.get(), rSel.get(), out.get());
+        std::vector<T> hb(2 * dim);
+        check(hipMemcpyAsync(hb.data(), out.get(), sizeof(T) * 2 * dim, hipMemcpyDeviceToHost, st_), "bounds");
+        check(hipStreamSynchronize(st_), "bounds");
+        root.mn.resize(dim);
+        root.mx.resize(dim);
+        for (int d = 0; d < dim; ++d) {
+            root.mn[d] = hb[2 * d];
+            root.mx[d] = hb[2 * d + 1];
+        }
+        calc_dist(root.mn, root.mx, root.dist);
+    }
+    root.level = 0;
+    root.ownBuf = 0;
+    root.ownOff = 0;
+    root.ownCnt = n;
+    tiles_.push_back(root);
+
+    std::vector<int> parents{0};
+    levels_ = 0;
+    DevBuf<int> zeroFlag;
+    zeroFlag.reserve(1);
+    {
+        int z = 0;
+        check(hipMemcpyAsync(zeroFlag.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "flag");
+    }
+    while (!parents.empty()) {
+        if (levels_ > 64) throw std::runtime_error("tile tree exceeds 64 levels (identical points?)");
+        // ---- parent descriptors and cell interval tables
+        std::vector<ParentDev<T>> pdev(parents.size());
+        std::vector<T> ivLo, ivHi;
+        std::vector<WorkItem> ownItems, nearItems;
+        std::vector<int> childBase(parents.size()), childCount(parents.size());
+        std::vector<std::vector<int>> nbinsOf(parents.size());
+        int64_t ownTotal = 0, posTotal = 0;
+        int totalChildren = 0;
+        for (size_t q = 0; q < parents.size(); ++q) {
+            const HTile& t = tiles_[parents[q]];
+            std::vector<int> nb(dim, 1);
+            if (parents[q] == 0)
+                for (int d = 0; d < dim; ++d) nb[d] = (int)tpd[d];
+            else
+                nb[t.level % dim] = 2;
+            nbinsOf[q] = nb;
+            ParentDev<T>& pd = pdev[q];
+            std::memset(&pd, 0, sizeof(pd));
+            pd.idx = bufs_[t.ownBuf]->get();
+            pd.off = t.ownOff;
+            pd.cnt = t.ownCnt;
+            pd.outOff = ownTotal;
+            pd.nbIdx = t.nbBuf >= 0 ? bufs_[t.nbBuf]->get() : nullptr;
+            pd.nbOff = t.nbOff;
+            pd.nbCnt = ext ? t.nbCnt : 0;
+            pd.posOff = posTotal;
+            pd.childBase = totalChildren;
+            pd.ivOff = (int)ivLo.size();
+            int stride = 1;
+            int nch = 1;
+            for (int d = 0; d < dim; ++d) {
+                pd.nbins[d] = nb[d];
+                pd.stride[d] = stride;
+                stride *= nb[d];
+                nch *= nb[d];
+                pd.mn[d] = t.mn[d];
+                pd.dist[d] = t.dist[d];
+            }
+            if (!local) {
+                // allocTiles (grouped_tile_tree.inl:144-211): [min + c*step, min + (c+1)*step),
+                // the last cell of a dimension ending at the parent max
+                for (int d = 0; d < dim; ++d) {
+                    const T step = t.dist[d] / static_cast<T>(nb[d]);
+                    for (int c = 0; c < nb[d]; ++c) {
+                        ivLo.push_back(t.mn[d] + c * step);
+                        ivHi.push_back(c == nb[d] - 1 ? t.mx[d] : t.mn[d] + (c + 1) * step);
+                    }
+                }
+            }
+            childBase[q] = totalChildren;
+            childCount[q] = nch;
+            totalChildren += nch;
+            for (int64_t s = 0; s < t.ownCnt; s += 2048)
+                ownItems.push_back({(int)q, (int)s, (int)std::min<int64_t>(t.ownCnt, s + 2048), 0});
+            const int64_t np = t.ownCnt + (ext ? t.nbCnt : 0);
+            for (int64_t s = 0; s < np; s += 2048)
+                nearItems.push_back({(int)q, (int)s, (int)std::min<int64_t>(np, s + 2048), 0});
+            ownTotal += t.ownCnt;
+            posTotal += np;
+        }
+        if (ownTotal >= (int64_t(1) << 31) || posTotal >= (int64_t(1) << 31))
+            throw std::runtime_error("tile level too large");
+        DevBuf<ParentDev<T>> dPar;
+        DevBuf<WorkItem> dOwnItems, dNearItems;
+        DevBuf<T> dLo, dHi;
+        dPar.reserve(pdev.size());
+        check(hipMemcpyAsync(dPar.get(), pdev.data(), sizeof(ParentDev<T>) * pdev.size(), hipMemcpyHostToDevice, st_),
+              "parents");
+        dOwnItems.reserve(std::max<size_t>(ownItems.size(), 1));
+        if (!ownItems.empty())
+            check(hipMemcpyAsync(dOwnItems.get(), ownItems.data(), sizeof(WorkItem) * ownItems.size(),
+                                 hipMemcpyHostToDevice, st_),
+                  "items");
+        dLo.reserve(std::max<size_t>(ivLo.size(), 1));
+        dHi.reserve(std::max<size_t>(ivHi.size(), 1));
+        if (!ivLo.empty()) {
+            check(hipMemcpyAsync(dLo.get(), ivLo.data(), sizeof(T) * ivLo.size(), hipMemcpyHostToDevice, st_), "iv");
+            check(hipMemcpyAsync(dHi.get(), ivHi.data(), sizeof(T) * ivHi.size(), hipMemcpyHostToDevice, st_), "iv");
+        }
+
+        // ---- own samples: bin + stable sort by child
+        DevBuf<int> keys, vals;
+        keys.reserve(std::max<int64_t>(ownTotal, 1));
+        vals.reserve(std::max<int64_t>(ownTotal, 1));
+        if (!ownItems.empty())
+            hipLaunchKernelGGL((bin_kernel<T>), dim3((unsigned)ownItems.size()), dim3(256), 0, st_, P, ps, dim,
+                               local ? 1 : 0, dPar.get(), dOwnItems.get(), dLo.get(), dHi.get(), keys.get(),
+                               vals.get());
+        check(hipGetLastError(), "bin");
+        const int levelOwnBuf = (int)bufs_.size();
+        bufs_.emplace_back(new DevBuf<int>());
+        int* sortedVals = bufs_.back()->reserve(std::max<int64_t>(ownTotal, 1));
+        DevBuf<int> sortedKeys;
+        sortedKeys.reserve(std::max<int64_t>(ownTotal, 1));
+        DevBuf<int64_t> dN, dM;
+        dN.reserve(1);
+        dM.reserve(1);
+        check(hipMemcpyAsync(dN.get(), &ownTotal, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
+        DevBuf<unsigned char> ws;
+        ws.reserve(sort_workspace_bytes(std::max<int64_t>(ownTotal, 1), std::max(totalChildren, 2)));
+        if (ownTotal > 0)
+            check(stable_sort_by_key(keys.get(), vals.get(), dN.get(), ownTotal, std::max(totalChildren, 2),
+                                     sortedKeys.get(), sortedVals, dM.get(), ws.get(), st_),
+                  "sort own");
+        DevBuf<int> cOff, cEnd;
+        cOff.reserve(totalChildren);
+        cEnd.reserve(totalChildren);
+        hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cOff.get(),
+                           (int64_t)totalChildren);
+        hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cEnd.get(),
+                           (int64_t)totalChildren);
+        if (ownTotal > 0)
+            check(launch_chain_bounds(sortedKeys.get(), dM.get(), ownTotal, cOff.get(), cEnd.get(), zeroFlag.get(), st_),
+                  "runs");
+        std::vector<int> hOff(totalChildren), hEnd(totalChildren);
+        check(hipMemcpyAsync(hOff.data(), cOff.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_), "runs");
+        check(hipMemcpyAsync(hEnd.data(), cEnd.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_), "runs");
+
+        // ---- EXTENDED: nearby samples of every child cell
+        std::vector<int> nOff(totalChildren, 0), nEnd(totalChildren, 0);
+        int levelNbBuf = -1;
+        if (ext && posTotal > 0) {
+            DevBuf<int> counts;
+            counts.reserve(posTotal + 1);
+            dNearItems.reserve(nearItems.size());
+            check(hipMemcpyAsync(dNearItems.get(), nearItems.data(), sizeof(WorkItem) * nearItems.size(),
+                                 hipMemcpyHostToDevice, st_),
+                  "items");
+            hipLaunchKernelGGL(zero_kernel, dim3(grid_of(posTotal + 1)), dim3(256), 0, st_, counts.get(), posTotal + 1);
+            hipLaunchKernelGGL((nearby_kernel<T, true>), dim3((unsigned)nearItems.size()), dim3(256), 0, st_, P, ps,
+                               dim, r1, dPar.get(), dNearItems.get(), dLo.get(), dHi.get(), counts.get(), nullptr,
+                               nullptr);
+            DevBuf<int> partials;
+            partials.reserve(scan_partials_count(posTotal + 1));
+            DevBuf<int64_t> pairTotal;
+            pairTotal.reserve(1);
+            check(exclusive_scan_inplace(counts.get(), posTotal + 1, partials.get(), pairTotal.get(), st_), "scan");
+            int64_t pairs = 0;
+            check(hipMemcpyAsync(&pairs, pairTotal.get(), sizeof(int64_t), hipMemcpyDeviceToHost, st_), "pairs");
+            check(hipStreamSynchronize(st_), "pairs");
+            if (pairs >= (int64_t(1) << 31)) throw std::runtime_error("too many neighbour samples");
+            levelNbBuf = (int)bufs_.size();
+            bufs_.emplace_back(new DevBuf<int>());
+            int* nbVals = bufs_.back()->reserve(std::max<int64_t>(pairs, 1));
+            if (pairs > 0) {
+                DevBuf<int> pk, pv, sk;
+                pk.reserve(pairs);
+                pv.reserve(pairs);
+                sk.reserve(pairs);
+                hipLaunchKernelGGL((nearby_kernel<T, false>), dim3((unsigned)nearItems.size()), dim3(256), 0, st_, P,
+                                   ps, dim, r1, dPar.get(), dNearItems.get(), dLo.get(), dHi.get(), counts.get(),
+                                   pk.get(), pv.get());
+                DevBuf<int64_t> pn, pm;
+                pn.reserve(1);
+                pm.reserve(1);
+                check(hipMemcpyAsync(pn.get(), &pairs, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
+                DevBuf<unsigned char> ws2;
+                ws2.reserve(sort_workspace_bytes(pairs, std::max(totalChildren, 2)));
+                check(stable_sort_by_key(pk.get(), pv.get(), pn.get(), pairs, std::max(totalChildren, 2), sk.get(), nbVals,
+                                         pm.get(), ws2.get(), st_),
+                      "sort nb");
+                hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cOff.get(),
+                                   (int64_t)totalChildren);
+                hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cEnd.get(),
+                                   (int64_t)totalChildren);
+                check(launch_chain_bounds(sk.get(), pm.get(), pairs, cOff.get(), cEnd.get(), zeroFlag.get(), st_),
+                      "runs");
+                check(hipMemcpyAsync(nOff.data(), cOff.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_),
+                      "runs");
+                check(hipMemcpyAsync(nEnd.data(), cEnd.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_),
+                      "runs");
+            }
+        }
+        check(hipStreamSynchronize(st_), "level");
+
+        // ---- LOCAL: tight bounds of the kept children
+        std::vector<int> keep;
+        for (int c = 0; c < totalChildren; ++c)
+            if (hEnd[c] - hOff[c] >= 2) keep.push_back(c);
+        std::vector<T> tb;
+        if (local && !keep.empty()) {
+            DevBuf<int> dSel;
+            dSel.reserve(keep.size());
+            check(hipMemcpyAsync(dSel.get(), keep.data(), sizeof(int) * keep.size(), hipMemcpyHostToDevice, st_), "sel");
+            DevBuf<T> out;
+            out.reserve(keep.size() * 2 * dim);
+            // cOff/cEnd were overwritten by the neighbour runs only for EXTENDED trees
+            check(hipMemcpyAsync(cOff.get(), hOff.data(), sizeof(int) * totalChildren, hipMemcpyHostToDevice, st_), "o");
+            check(hipMemcpyAsync(cEnd.get(), hEnd.data(), sizeof(int) * totalChildren, hipMemcpyHostToDevice, st_), "o");
+            hipLaunchKernelGGL((seg_bounds_kernel<T>), dim3((unsigned)keep.size()), dim3(256), 0, st_, P, ps, dim,
+                               sortedVals, cOff.get(), cEnd.get(), dSel.get(), out.get());
+            tb.resize(keep.size() * 2 * dim);
+            check(hipMemcpyAsync(tb.data(), out.get(), sizeof(T) * tb.size(), hipMemcpyDeviceToHost, st_), "bounds");
+            check(hipStreamSynchronize(st_), "bounds");
+        }
+
+        // ---- children (in each parent's linear order), leaf / split decisions
+        std::vector<int> next;
+        size_t kp = 0;
+        for (size_t q = 0; q < parents.size(); ++q) {
+            const int pt = parents[q];
+            const int lvl = tiles_[pt].level;
+            const std::vector<int>& nb = nbinsOf[q];
+            for (int b = 0; b < childCount[q]; ++b) {
+                const int c = childBase[q] + b;
+                const int64_t cnt = hEnd[c] - hOff[c];
+                if (cnt < 2) continue;
+                HTile ch;
+                ch.level = lvl + 1;
+                ch.mn.resize(dim);
+                ch.mx.resize(dim);
+                if (local) {
+                    for (int d = 0; d < dim; ++d) {
+                        ch.mn[d] = tb[kp * 2 * dim + 2 * d];
+                        ch.mx[d] = tb[kp * 2 * dim + 2 * d + 1];
+                    }
+                    ++kp;
+                } else {
+                    int rem = b, o = pdev[q].ivOff;
+                    for (int d = 0; d < dim; ++d) {
+                        const int cd = rem % nb[d];
+                        rem /= nb[d];
+                        ch.mn[d] = ivLo[o + cd];
+                        ch.mx[d] = ivHi[o + cd];
+                        o += nb[d];
+                    }
+                }
+                calc_dist(ch.mn, ch.mx, ch.dist);
+                ch.ownBuf = levelOwnBuf;
+                ch.ownOff = hOff[c];
+                ch.ownCnt = cnt;
+                if (ext) {
+                    ch.nbBuf = levelNbBuf;
+                    ch.nbOff = nOff[c];
+                    ch.nbCnt = nEnd[c] - nOff[c];
+                }
+                ch.leaf = cnt < cap;
+                tiles_.push_back(ch);
+                const int id = (int)tiles_.size() - 1;
+                if (pt == 0)
+                    rootChildren_.push_back(id);
+                else
+                    tiles_[pt].children.push_back(id);
+                if (!ch.leaf) next.push_back(id);
+            }
+        }
+        for (int pt : parents)
+            if (pt != 0 && tiles_[pt].children.empty())
+                throw std::runtime_error("a split tile lost all its samples");
+        parents.swap(next);
+        ++levels_;
+    }
+
+    // ---- leaves in preorder
+    for (int c : rootChildren_) preorder(c);
+    const int L = (int)leafTiles_.size();
+    leaves_.resize(L);
+    totalPositions_ = 0;
+    for (int l = 0; l < L; ++l) {
+        const HTile& t = tiles_[leafTiles_[l]];
+        LeafInfo<T>& li = leaves_[l];
+        li.ownCnt = t.ownCnt;
+        li.mn = t.mn;
+        li.mx = t.mx;
+        li.sphereCap = count_spheres(t.dist, r1);
+        li.nbCnt = ext ? t.nbCnt : 0;
+    }
+    if (kind == TREE_GROUPED) {
+        for (int l = 0; l < L; ++l)
+            for (int j = 0; j < L; ++j)
+                if (j != l && boxes_overlap(leaves_[l].mn, leaves_[l].mx, leaves_[j].mn, leaves_[j].mx)) {
+                    leaves_[l].nbLeaves.push_back(j);
+                    leaves_[l].nbCnt += leaves_[j].ownCnt;
+                }
+    }
+    for (auto& li : leaves_) totalPositions_ += li.ownCnt + li.nbCnt;
+    if (totalPositions_ >= (int64_t(1) << 31)) throw std::runtime_error("tiled layout exceeds 2^31 samples");
+}
+
+template <typename T>
+void TileTreeBuilder<T>::writeIndex(int* dIndex) {
+    std::vector<RangeCopy> rc;
+    int64_t dst = 0;
+    const int L = (int)leaves_.size();
+    for (int l = 0; l < L; ++l) {
+        const HTile& t = tiles_[leafTiles_[l]];
+        rc.push_back({bufs_[t.ownBuf]->get(), t.ownOff, t.ownCnt, dst});
+        dst += t.ownCnt;
+        if (kind_ == TREE_EXTENDED && t.nbCnt > 0) {
+            rc.push_back({bufs_[t.nbBuf]->get(), t.nbOff, t.nbCnt, dst});
+            dst += t.nbCnt;
+        }
+        if (kind_ == TREE_GROUPED)
+            for (int j : leaves_[l].nbLeaves) {
+                const HTile& o = tiles_[leafTiles_[j]];
+                rc.push_back({bufs_[o.ownBuf]->get(), o.ownOff, o.ownCnt, dst});
+                dst += o.ownCnt;
+            }
+    }
+    if (rc.empty()) return;
+    DevBuf<RangeCopy> d;
+    d.reserve(rc.size());
+    check(hipMemcpyAsync(d.get(), rc.data(), sizeof(RangeCopy) * rc.size(), hipMemcpyHostToDevice, st_), "ranges");
+    hipLaunchKernelGGL(copy_ranges_kernel, dim3((unsigned)rc.size()), dim3(256), 0, st_, d.get(), dIndex);
+    check(hipGetLastError(), "copy_ranges");
+    check(hipStreamSynchronize(st_), "writeIndex");
+}
+
+template class TileTreeBuilder<float>;
+template class TileTreeBuilder<double>;
+
+}  // namespace rdhip
