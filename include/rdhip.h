@@ -117,6 +117,21 @@ int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double
                                 int64_t max_tile_capacity, const int64_t* tiles_per_dim,
                                 int refinement, int64_t max_evolve_iters, int64_t* out_count);
 
+/* Builds a tile tree (enum rdhip_tree_kind) over the current points and keeps it for
+ * rdhip_get_tree: the device counterpart of TiledLocalTree / TiledGroupTree::buildTree
+ * (rd/cpu/tiled/local_tile_tree.inl:48-76, grouped_tile_tree.inl:86-115, 528-548).
+ * out_leaves receives the number of leaves. */
+int rdhip_build_tree(rdhip_ctx* ctx, int kind, double r1, int64_t max_tile_capacity,
+                     const int64_t* tiles_per_dim, int64_t* out_leaves);
+
+/* Leaves of the last built tree in preorder (host outputs, optional):
+ *   leaf_info[4*l..4*l+3] = {own samples, neighbours, chosen capacity, 0}; neighbours counts
+ *     samples for EXTENDED trees and leaves for GROUPED trees;
+ *   own_idx: every leaf's own input indices, concatenated (in the tile's sample order);
+ *   nb_idx: EXTENDED: neighbour input indices; GROUPED: neighbour leaf positions;
+ *   bounds: 2*dim values per leaf, [min_0, max_0, min_1, max_1, ...] in the value type. */
+int rdhip_get_tree(rdhip_ctx* ctx, int64_t* leaf_info, int32_t* own_idx, int32_t* nb_idx, void* bounds);
+
 /* Sets a tuning / instrumentation option (enum rdhip_option). */
 int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value);
 

@@ -29,8 +29,9 @@ EXPORTS = [
     "rdhip_version", "rdhip_last_error", "rdhip_create", "rdhip_destroy", "rdhip_set_points",
     "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
-    "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed",
+    "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
 ]
+TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE = 0, 1, 2
 
 
@@ -71,6 +72,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_decimate": (i32, [vp, C.c_double, i32, vp, C.POINTER(i64)]),
         "rdhip_set_option": (i32, [vp, i32, i64]),
         "rdhip_evolve_fixed": (i32, [vp, C.c_double, i64]),
+        "rdhip_build_tree": (i32, [vp, i32, C.c_double, i64, vp, C.POINTER(i64)]),
+        "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -164,6 +167,29 @@ class Context:
                                                          int(max_tile_capacity), _ptr(tpd), int(bool(refinement)),
                                                          int(max_evolve_iters), C.byref(k)))
         return self.points()
+
+    def tree(self, kind, r1, max_tile_capacity, tiles_per_dim):
+        """Builds a tile tree on the device; returns its leaves in preorder as dicts
+        (own, nb, cap, bounds) like the oracle's tree dump."""
+        tpd = np.ascontiguousarray(tiles_per_dim, dtype=np.int64)
+        L = C.c_int64()
+        self._check(self.lib.rdhip_build_tree(self.h, int(kind), float(r1), int(max_tile_capacity), _ptr(tpd),
+                                              C.byref(L)))
+        L = L.value
+        info = np.zeros((max(L, 1), 4), dtype=np.int64)
+        self._check(self.lib.rdhip_get_tree(self.h, _ptr(info), None, None, None))
+        own = np.zeros(max(int(info[:L, 0].sum()), 1), dtype=np.int32)
+        nb = np.zeros(max(int(info[:L, 1].sum()), 1), dtype=np.int32)
+        bounds = np.zeros((max(L, 1), 2 * self.dim), dtype=self.dtype)
+        self._check(self.lib.rdhip_get_tree(self.h, _ptr(info), _ptr(own), _ptr(nb), _ptr(bounds)))
+        leaves, o, b = [], 0, 0
+        for l in range(L):
+            no, nn, cap, _ = info[l]
+            leaves.append(dict(own=own[o:o + no].astype(np.int64), nb=nb[b:b + nn].astype(np.int64), cap=int(cap),
+                               bounds=bounds[l].reshape(self.dim, 2).copy()))
+            o += no
+            b += nn
+        return leaves
 
     # -- phases --------------------------------------------------------------------
     def choose(self, r):

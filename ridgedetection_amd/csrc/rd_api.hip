@@ -2,6 +2,7 @@
 #include "../../include/rdhip.h"
 #include "rd_engine.hpp"
 #include "rd_tiled.hpp"
+#include "rd_tree.hpp"
 
 #include <memory>
 #include <new>
@@ -20,6 +21,12 @@ struct rdhip_ctx {
     bool hasPoints = false;
     bool hasChosen = false;
     Timers timers;
+    std::unique_ptr<TileTreeBuilder<float>> tree32;
+    std::unique_ptr<TileTreeBuilder<double>> tree64;
+    int treeKind = -1;
+
+    std::unique_ptr<TileTreeBuilder<float>>& tree(float) { return tree32; }
+    std::unique_ptr<TileTreeBuilder<double>>& tree(double) { return tree64; }
 };
 
 namespace {
@@ -206,6 +213,74 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->scan_kernel_ms = e.timers.scanKernelMs;
         out->scan_launches = e.timers.scanLaunches;
         out->iteration_ms = e.timers.iterationMs;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_build_tree(rdhip_ctx* ctx, int kind, double r1, int64_t max_tile_capacity, const int64_t* tiles_per_dim,
+                     int64_t* out_leaves) {
+    if (kind < RDHIP_TREE_LOCAL || kind > RDHIP_TREE_EXTENDED) return fail(RDHIP_ERR_INVALID, "unknown tree kind");
+    if (!(r1 > 0)) return fail(RDHIP_ERR_INVALID, "r1 must be positive");
+    if (max_tile_capacity < 3) return fail(RDHIP_ERR_INVALID, "max_tile_capacity must be at least 3");
+    if (ctx && !tiles_per_dim) return fail(RDHIP_ERR_INVALID, "null tiles_per_dim");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasPoints || e.numPoints() < 1) return fail(RDHIP_ERR_STATE, "no input points");
+        std::vector<int64_t> tpd(tiles_per_dim, tiles_per_dim + ctx->dim);
+        for (int64_t v : tpd)
+            if (v < 1 || v > 4096) return fail(RDHIP_ERR_INVALID, "tiles per dimension must be in [1, 4096]");
+        auto& tb = ctx->tree(zero);
+        tb.reset(new TileTreeBuilder<T>(ctx->dim, ctx->stream));
+        tb->build(e.devicePoints(), e.pointStride(), e.numPoints(), kind, (T)r1, max_tile_capacity, tpd);
+        ctx->treeKind = kind;
+        if (out_leaves) *out_leaves = (int64_t)tb->leaves().size();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_get_tree(rdhip_ctx* ctx, int64_t* leaf_info, int32_t* own_idx, int32_t* nb_idx, void* bounds) {
+    return withEngine(ctx, [&](auto&, auto zero) -> int {
+        using T = decltype(zero);
+        auto& tb = ctx->tree(zero);
+        if (!tb) return fail(RDHIP_ERR_STATE, "no tree (call rdhip_build_tree first)");
+        const auto& leaves = tb->leaves();
+        const int64_t total = tb->totalPositions();
+        std::vector<int> idx((size_t)std::max<int64_t>(total, 1));
+        if (total > 0) {
+            DevBuf<int> d;
+            d.reserve(total);
+            tb->writeIndex(d.get());
+            check(hipMemcpy(idx.data(), d.get(), sizeof(int) * total, hipMemcpyDeviceToHost), "tree index");
+        }
+        int64_t pos = 0, o = 0, b = 0;
+        for (size_t l = 0; l < leaves.size(); ++l) {
+            const auto& li = leaves[l];
+            const int64_t nbShown = ctx->treeKind == RDHIP_TREE_GROUPED ? (int64_t)li.nbLeaves.size() : li.nbCnt;
+            if (leaf_info) {
+                leaf_info[4 * l] = li.ownCnt;
+                leaf_info[4 * l + 1] = nbShown;
+                leaf_info[4 * l + 2] = (int64_t)li.sphereCap;
+                leaf_info[4 * l + 3] = 0;
+            }
+            if (own_idx)
+                for (int64_t i = 0; i < li.ownCnt; ++i) own_idx[o + i] = idx[pos + i];
+            o += li.ownCnt;
+            if (nb_idx) {
+                if (ctx->treeKind == RDHIP_TREE_GROUPED)
+                    for (size_t j = 0; j < li.nbLeaves.size(); ++j) nb_idx[b + j] = li.nbLeaves[j];
+                else
+                    for (int64_t i = 0; i < li.nbCnt; ++i) nb_idx[b + i] = idx[pos + li.ownCnt + i];
+            }
+            b += nbShown;
+            if (bounds) {
+                T* bo = static_cast<T*>(bounds) + 2 * ctx->dim * l;
+                for (int d = 0; d < ctx->dim; ++d) {
+                    bo[2 * d] = li.mn[d];
+                    bo[2 * d + 1] = li.mx[d];
+                }
+            }
+            pos += li.ownCnt + li.nbCnt;
+        }
         return RDHIP_OK;
     });
 }
