@@ -90,7 +90,11 @@ typedef struct rdhip_timers {
 enum rdhip_option {
     RDHIP_OPT_PPT = 0,             /* samples per lane of the evolve scan: 1, 2 or 4 (default 4) */
     RDHIP_OPT_ITERS_PER_SYNC = 1,  /* evolve iterations queued between convergence checks (8) */
-    RDHIP_OPT_PROFILE = 2          /* 1: time the scan kernel and iterations with HIP events */
+    RDHIP_OPT_PROFILE = 2,         /* 1: time the scan kernel and iterations with HIP events */
+    RDHIP_OPT_PERSISTENT = 3,      /* 1 (default): persistent scan blocks fed by a device work
+                                      queue; 0: one block per work item */
+    RDHIP_OPT_GRAPH = 4            /* 1 (default): replay captured batches of evolve iterations
+                                      (hipGraph) until the device convergence flag is set */
 };
 
 int rdhip_version(void);
@@ -109,6 +113,16 @@ int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, 
  * chosen points is stable (rd/cpu/brute_force/ridge_detection.inl:67-146). */
 int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evolve_iters,
                           int64_t* out_count);
+
+/* Batched brute-force ridge detection: `num_clouds` independent clouds of the context's
+ * dimension (host ROW_MAJOR pointers, sizes[c] points, radii r1[c] / r2[c]) each run the
+ * loop of rdhip_ridge_detection, all clouds of a phase in one launch.  out_counts[c]
+ * receives each cloud's number of ridge points; rdhip_get_points then returns all clouds'
+ * points concatenated in cloud order.  No reference counterpart: the reference runs one
+ * cloud per RidgeDetection object. */
+int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds,
+                                const int64_t* sizes, const double* r1, const double* r2,
+                                int64_t max_evolve_iters, int64_t* out_counts);
 
 /* Tiled ridge detection (tree build, per-tile choose/evolve, local or global decimation,
  * optional end-phase refinement: decimate + evolve over all points). tiles_per_dim has dim

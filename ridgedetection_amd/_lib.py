@@ -30,9 +30,10 @@ EXPORTS = [
     "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
+    "rdhip_batch_ridge_detection",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
-OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE = 0, 1, 2
+OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH = 0, 1, 2, 3, 4
 
 
 class Timers(C.Structure):
@@ -74,6 +75,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_evolve_fixed": (i32, [vp, C.c_double, i64]),
         "rdhip_build_tree": (i32, [vp, i32, C.c_double, i64, vp, C.POINTER(i64)]),
         "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
+        "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -159,6 +161,26 @@ class Context:
         self._check(self.lib.rdhip_ridge_detection(self.h, float(r1), float(r2), int(max_evolve_iters),
                                                    C.byref(k)))
         return self.points()
+
+    def batch(self, clouds, r1, r2, max_evolve_iters=0):
+        """Batched ridge detection over clouds (list of (n_c, dim) arrays) with per-cloud radii.
+        Returns the list of per-cloud ridge points."""
+        arrs = [np.ascontiguousarray(c, dtype=self.dtype) for c in clouds]
+        m = len(arrs)
+        ptrs = (C.c_void_p * m)(*[a.ctypes.data for a in arrs])
+        sizes = np.array([len(a) for a in arrs], dtype=np.int64)
+        vr1 = np.ascontiguousarray(r1, dtype=np.float64).reshape(m)
+        vr2 = np.ascontiguousarray(r2, dtype=np.float64).reshape(m)
+        counts = np.zeros(m, dtype=np.int64)
+        self._check(self.lib.rdhip_batch_ridge_detection(self.h, m, ptrs, _ptr(sizes), _ptr(vr1), _ptr(vr2),
+                                                         int(max_evolve_iters), _ptr(counts)))
+        self.n = int(sizes.sum())
+        allp = self.points()
+        out, o = [], 0
+        for k in counts:
+            out.append(allp[o:o + k].copy())
+            o += k
+        return out
 
     def tiled(self, algorithm, r1, r2, max_tile_capacity, tiles_per_dim, refinement=False, max_evolve_iters=0):
         tpd = np.ascontiguousarray(tiles_per_dim, dtype=np.int64)

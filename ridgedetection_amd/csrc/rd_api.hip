@@ -4,6 +4,7 @@
 #include "rd_tiled.hpp"
 #include "rd_tree.hpp"
 
+#include <cstring>
 #include <memory>
 #include <new>
 #include <string>
@@ -139,6 +140,55 @@ int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evol
         e.ridgeDetectionLocal(opt);
         ctx->hasChosen = true;
         if (out_count) *out_count = e.totalCount();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, const int64_t* sizes,
+                                const double* r1, const double* r2, int64_t max_evolve_iters, int64_t* out_counts) {
+    if (num_clouds < 1 || num_clouds > (int64_t(1) << 24)) return fail(RDHIP_ERR_INVALID, "cloud count out of range");
+    if (ctx && (!clouds || !sizes || !r1 || !r2)) return fail(RDHIP_ERR_INVALID, "null batch arrays");
+    if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        const int dim = ctx->dim;
+        int64_t total = 0;
+        for (int64_t c = 0; c < num_clouds; ++c) {
+            if (sizes[c] < 1 || !clouds[c]) return fail(RDHIP_ERR_INVALID, "every cloud needs at least one point");
+            if (!(r1[c] >= 0) || !(r2[c] >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
+            total += sizes[c];
+        }
+        if (total >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "batch exceeds 2^31 points");
+        std::vector<T> all((size_t)total * dim);
+        std::vector<SegDesc> segs((size_t)num_clouds);
+        std::vector<int64_t> caps((size_t)num_clouds);
+        std::vector<T> vr1((size_t)num_clouds), vr2((size_t)num_clouds);
+        int64_t off = 0;
+        for (int64_t c = 0; c < num_clouds; ++c) {
+            std::memcpy(all.data() + off * dim, clouds[c], sizeof(T) * sizes[c] * dim);
+            SegDesc s{};
+            s.pOff = off;
+            s.nP = (int32_t)sizes[c];
+            s.nOwn = (int32_t)sizes[c];
+            s.group = (int32_t)c;
+            segs[c] = s;
+            caps[c] = sizes[c];
+            vr1[c] = (T)r1[c];
+            vr2[c] = (T)r2[c];
+            off += sizes[c];
+        }
+        e.timers = Timers();
+        e.setPoints(all.data(), total, RDHIP_ROW_MAJOR, 0, false);
+        ctx->hasPoints = true;
+        e.setSegments(segs, caps, vr1, vr2, false);
+        RunOptions opt = e.options;
+        opt.maxEvolveIters = max_evolve_iters;
+        e.ridgeDetectionLocal(opt);
+        ctx->hasChosen = true;
+        if (out_counts) {
+            std::vector<int> K = e.countsHost();
+            for (int64_t c = 0; c < num_clouds; ++c) out_counts[c] = K[c];
+        }
         return RDHIP_OK;
     });
 }
@@ -302,6 +352,12 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
             case RDHIP_OPT_PROFILE:
                 e.options.profile = value != 0;
                 return RDHIP_OK;
+            case RDHIP_OPT_PERSISTENT:
+                e.options.persistent = value != 0;
+                return RDHIP_OK;
+            case RDHIP_OPT_GRAPH:
+                e.options.graph = value != 0;
+                return RDHIP_OK;
         }
         return fail(RDHIP_ERR_INVALID, "unknown option");
     });
@@ -345,7 +401,7 @@ int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layo
         using T = decltype(zero);
         if (!ctx->hasPoints) return fail(RDHIP_ERR_STATE, "no input points");
         if (count > e.numPoints() + 4096) return fail(RDHIP_ERR_INVALID, "more chosen points than capacity");
-        if (e.numSegments() != 1) e.setSingleSegment(T(0), T(0));
+        e.setSingleSegment(T(0), T(0));
         e.setChosen(0, static_cast<const T*>(chosen), count, layout);
         ctx->hasChosen = true;
         return RDHIP_OK;

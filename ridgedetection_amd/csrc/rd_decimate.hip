@@ -96,7 +96,9 @@ __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __re
     if (!fill) {
         a.c1[e] = c1;
         a.c2[e] = c2;
-        a.rowOff[e] = c2 - 1;   // neighbours other than itself; scanned to offsets
+        // neighbours other than itself (scanned to offsets); elements of a skipped group
+        // (n == 0) have no list at all
+        a.rowOff[e] = n > 0 ? c2 - 1 : 0;
     }
 }
 

@@ -56,6 +56,20 @@ __global__ void gather_chosen_kernel(const T* __restrict__ S, int64_t sStride, c
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384)); }
 
+// copies every segment's first `cnt` slots (coordinates and chosen indices) to a new layout
+template <typename T>
+__global__ void relayout_slots_kernel(const T* __restrict__ S, int64_t sStride, const int* __restrict__ idx,
+                                      const int64_t* __restrict__ oldOff, const int64_t* __restrict__ newOff,
+                                      const int* __restrict__ cnt, int dim, T* __restrict__ S2, int64_t s2Stride,
+                                      int* __restrict__ idx2) {
+    const int s = blockIdx.x;
+    const int64_t o = oldOff[s], n = newOff[s];
+    for (int k = threadIdx.x; k < cnt[s]; k += 256) {
+        for (int d = 0; d < dim; ++d) S2[d * s2Stride + n + k] = S[d * sStride + o + k];
+        idx2[n + k] = idx[o + k];
+    }
+}
+
 }  // namespace
 
 template <typename T>
@@ -201,7 +215,8 @@ void Engine<T>::setRadii(const std::vector<T>& r1, const std::vector<T>& r2) {
 
 template <typename T>
 void Engine<T>::buildWorkItems() {
-    std::vector<WorkItem> items;
+    std::vector<WorkItem>& items = items_;
+    items.clear();
     const int per = kScanThreads * std::max(1, options.ppt);
     for (int s = 0; s < (int)segs_.size(); ++s)
         for (int st = 0; st < segs_[s].nP; st += per) items.push_back({s, st, std::min(segs_[s].nP, st + per), 0});
@@ -213,6 +228,31 @@ void Engine<T>::buildWorkItems() {
     // choose filter items for the chunked (large segment) path
     for (auto* b : chooseItems_) delete b;
     chooseItems_.clear();
+}
+
+// Longest-first order for the persistent scan's work queue: cost ~ samples x live chosen.
+template <typename T>
+void Engine<T>::orderItemsByCost() {
+    if (items_.size() < 2) return;
+    std::vector<int> K = countsHost();
+    std::stable_sort(items_.begin(), items_.end(), [&](const WorkItem& a, const WorkItem& b) {
+        return (int64_t)(a.end - a.start) * K[a.seg] > (int64_t)(b.end - b.start) * K[b.seg];
+    });
+    check(hipMemcpyAsync(dItems_.get(), items_.data(), sizeof(WorkItem) * items_.size(), hipMemcpyHostToDevice, st_),
+          "items");
+    check(hipStreamSynchronize(st_), "items");
+}
+
+template <typename T>
+int Engine<T>::persistentBlocks() {
+    if (!options.persistent) return 0;
+    if (persistentBlocks_ == 0) {
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+        persistentBlocks_ = std::max(1, cus) * 8;
+    }
+    return persistentBlocks_;
 }
 
 template <typename T>
@@ -245,6 +285,56 @@ template <typename T>
 void Engine<T>::fillSentinels() {
     launch_fill_sentinel<T>(S_.get(), sStride_, dim_, dSegs_.get(), dSegK_.get(), (int)segs_.size(), st_);
     check(hipGetLastError(), "fill_sentinel");
+}
+
+template <typename T>
+void Engine<T>::compactSlots() {
+    const int ns = (int)segs_.size();
+    if (ns == 0) return;
+    std::vector<int> K = countsHost();
+    std::vector<int64_t> oldOff(ns), newOff(ns);
+    std::vector<int> copyCnt(ns);
+    int64_t off = 0;
+    for (int s = 0; s < ns; ++s) {
+        oldOff[s] = segs_[s].sOff;
+        newOff[s] = off;
+        const int64_t cap = round_up(std::max<int64_t>(K[s], 1), kChosenTile);
+        copyCnt[s] = (int)std::min<int64_t>(cap, segs_[s].sCap);   // live points + sentinel tail
+        segs_[s].sOff = off;
+        segs_[s].sCap = (int32_t)cap;
+        off += cap;
+    }
+    if (off == totalSlots_) return;
+    const int64_t newStride = round_up(std::max<int64_t>(off, 1), 64);
+    DevBuf<T> S2;
+    DevBuf<int> idx2;
+    S2.reserve((size_t)newStride * dim_);
+    idx2.reserve((size_t)std::max<int64_t>(off, 1));
+    DevBuf<int64_t> dOld, dNew;
+    DevBuf<int> dCnt;
+    dOld.reserve(ns);
+    dNew.reserve(ns);
+    dCnt.reserve(ns);
+    check(hipMemcpyAsync(dOld.get(), oldOff.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st_), "relayout");
+    check(hipMemcpyAsync(dNew.get(), newOff.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st_), "relayout");
+    check(hipMemcpyAsync(dCnt.get(), copyCnt.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "relayout");
+    hipLaunchKernelGGL((relayout_slots_kernel<T>), dim3(ns), dim3(256), 0, st_, S_.get(), sStride_, chosenIdx_.get(),
+                       dOld.get(), dNew.get(), dCnt.get(), dim_, S2.get(), newStride, idx2.get());
+    check(hipGetLastError(), "relayout");
+    check(hipStreamSynchronize(st_), "relayout");
+    S_.swap(S2);
+    chosenIdx_.swap(idx2);
+    totalSlots_ = off;
+    sStride_ = newStride;
+    std::vector<int> slotSeg((size_t)totalSlots_);
+    for (int s = 0; s < ns; ++s)
+        std::fill(slotSeg.begin() + segs_[s].sOff, slotSeg.begin() + segs_[s].sOff + segs_[s].sCap, s);
+    check(hipMemcpyAsync(dSegs_.get(), segs_.data(), sizeof(SegDesc) * ns, hipMemcpyHostToDevice, st_), "segs");
+    check(hipMemcpyAsync(dSlotSeg_.get(), slotSeg.data(), sizeof(int) * totalSlots_, hipMemcpyHostToDevice, st_),
+          "slots");
+    sums_.reserve((size_t)sStride_ * dim_);
+    check(hipStreamSynchronize(st_), "relayout");
+    updateKeyBound();
 }
 
 template <typename T>
@@ -319,7 +409,7 @@ void Engine<T>::choose() {
 template <typename T>
 void Engine<T>::enqueueIteration(bool record, int slot) {
     ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
-                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get()};
+                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3};
     const bool ev = options.profile && slot >= 0;
     if (ev) {
         while ((int)events_.size() < 3 * (slot + 1)) {
@@ -329,7 +419,7 @@ void Engine<T>::enqueueIteration(bool record, int slot) {
         }
         check(hipEventRecord(events_[3 * slot], st_), "event");
     }
-    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, st_), "evolve_scan");
+    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, persistentBlocks(), st_), "evolve_scan");
     if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
     const int64_t totalP = totalP_;
     // labels are live slots < keyBound_ (counts only shrink between refreshes)
@@ -372,15 +462,59 @@ void Engine<T>::evolveFixed(long long iters) {
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
     const int batch = std::max(1, options.itersPerSync);
     long long done = 0;
+    hipGraphExec_t exec = nullptr;
+    const bool useGraph = options.graph && !options.profile && iters >= batch &&
+                          captureIterations(batch, false, 0, &exec);
     while (done < iters) {
         const int nb = (int)std::min<long long>(batch, iters - done);
-        for (int b = 0; b < nb; ++b) enqueueIteration(false, b);
+        if (useGraph && nb == batch)
+            check(hipGraphLaunch(exec, st_), "hipGraphLaunch");
+        else
+            for (int b = 0; b < nb; ++b) enqueueIteration(false, b);
         check(hipStreamSynchronize(st_), "evolveFixed");
         if (options.profile) readEvents(nb);
         done += nb;
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T>
+bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGraphExec_t* exec) {
+    *exec = nullptr;
+    (void)persistentBlocks();   // device queries stay outside the capture
+    if (hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    bool ok = true;
+    try {
+        for (int b = 0; b < batch; ++b) {
+            enqueueIteration(false, -1);
+            if (finalize)
+                check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(),
+                                             capIter, flags_.get(), flags_.get() + 1, st_),
+                      "finalize");
+        }
+    } catch (const HipError&) {
+        ok = false;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st_, &g);
+    if (!ok || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return false;
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
 }
 
 template <typename T>
@@ -402,22 +536,32 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
     const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
     long long iters = 0;
+    const int batch = std::max(1, options.itersPerSync);
+    // the iterations after convergence in a batch return at once (device all-done flag)
+    hipGraphExec_t exec = nullptr;
+    const bool useGraph = options.graph && !options.profile && captureIterations(batch, true, capIter, &exec);
     for (;;) {
-        const int batch = std::max(1, options.itersPerSync);
-        for (int b = 0; b < batch; ++b) {
-            enqueueIteration(false, b);
-            check(launch_evolve_finalize(ns, dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
-                                         flags_.get(), flags_.get() + 1, st_),
-                  "finalize");
+        if (useGraph) {
+            check(hipGraphLaunch(exec, st_), "hipGraphLaunch");
+        } else {
+            for (int b = 0; b < batch; ++b) {
+                enqueueIteration(false, b);
+                check(launch_evolve_finalize(ns, dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
+                                             flags_.get(), flags_.get() + 1, st_),
+                      "finalize");
+            }
         }
         check(hipMemcpyAsync(f, flags_.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "flags");
         check(hipStreamSynchronize(st_), "evolve");
         if (options.profile) readEvents(batch);
         iters = f[1];
         if (f[0]) break;
-        if (iters >= kEvolveGuard)
+        if (iters >= kEvolveGuard) {
+            if (exec) (void)hipGraphExecDestroy(exec);
             throw std::runtime_error("evolve did not converge within the iteration guard");
+        }
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
     lastEvolveIters = iters;
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -538,6 +682,8 @@ void Engine<T>::ridgeDetectionLocal(const RunOptions& opt) {
     auto t0 = std::chrono::steady_clock::now();
     const int ns = (int)segs_.size();
     choose();
+    compactSlots();
+    orderItemsByCost();
     std::vector<int> K = countsHost();
     std::vector<char> active(ns);
     std::vector<int> skip(ns);
@@ -572,6 +718,8 @@ void Engine<T>::ridgeDetectionMixed(const RunOptions& opt) {
     auto t0 = std::chrono::steady_clock::now();
     const int ns = (int)segs_.size();
     choose();
+    compactSlots();
+    orderItemsByCost();
     std::vector<int> K = countsHost();
     int64_t total = std::accumulate(K.begin(), K.end(), int64_t(0));
     int64_t oldTotal = 0;

@@ -48,6 +48,10 @@ struct DevBuf {
         return p;
     }
     T* get() const { return p; }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
 };
 
 enum Layout { LAYOUT_ROW_MAJOR = 0, LAYOUT_COL_MAJOR = 1, LAYOUT_SOA = 2 };
@@ -66,6 +70,8 @@ struct RunOptions {
     int ppt = 4;                    // samples per lane in the evolve scan
     int itersPerSync = 8;           // evolve iterations enqueued between convergence checks
     bool profile = false;           // record HIP events around the scan kernel / iterations
+    bool persistent = true;         // scan as persistent blocks fed by a device work queue
+    bool graph = true;              // replay captured batches of iterations (hipGraph)
 };
 
 template <typename T>
@@ -105,6 +111,10 @@ public:
 
     // -- phases --------------------------------------------------------------------
     void choose();
+    // Re-lays the chosen slots so that every segment's capacity is its live count (rounded
+    // to a screening tile): counts only shrink after choose, and a dense slot range keeps
+    // the per-iteration sort to few radix passes when many segments are batched.
+    void compactSlots();
     // Runs evolve iterations on every segment whose flag in `active` is set (all when
     // empty) until each converges or reaches maxIter.  Returns the number of iterations.
     long long evolve(long long maxIter, const std::vector<char>& active = {});
@@ -176,6 +186,13 @@ private:
     const int* segSkip_ = nullptr;   // optional per-segment skip mask for decimate
 
     void buildWorkItems();
+    void orderItemsByCost();
+    std::vector<WorkItem> items_;
+    int persistentBlocks_ = 0;
+    int persistentBlocks();
+    // captures `batch` iterations (+ finalize when `finalize`) into an executable graph;
+    // returns false when the stream cannot be captured
+    bool captureIterations(int batch, bool finalize, int capIter, hipGraphExec_t* exec);
     void ensureEvolveScratch();
     void enqueueIteration(bool record, int slot = -1);
     void readEvents(int used);

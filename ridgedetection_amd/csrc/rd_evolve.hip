@@ -55,9 +55,7 @@ __device__ __forceinline__ double tile_min<double>(double m, double a, double b)
 }
 
 template <typename T, int DIM, int PPT>
-__global__ __launch_bounds__(kScanThreads) void evolve_scan_kernel(ScanArgs<T> a) {
-    if (*a.allDone) return;
-    const WorkItem it = a.items[blockIdx.x];
+__device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem it) {
     const SegDesc sd = a.segs[it.seg];
     const int tid = threadIdx.x;
     int* __restrict__ lab = a.labels + sd.pOff;
@@ -181,6 +179,35 @@ __global__ __launch_bounds__(kScanThreads) void evolve_scan_kernel(ScanArgs<T> a
 #pragma unroll
     for (int j = 0; j < PPT; ++j)
         if (idx[j] < it.end) lab[idx[j]] = label[j];
+}
+
+// one block per work item
+template <typename T, int DIM, int PPT>
+__global__ __launch_bounds__(kScanThreads) void evolve_scan_kernel(ScanArgs<T> a) {
+    if (*a.allDone) return;
+    scan_item<T, DIM, PPT>(a, a.items[blockIdx.x]);
+}
+
+// persistent blocks pulling work items from a device queue (items are ordered by
+// decreasing cost, so the longest segments start first)
+template <typename T, int DIM, int PPT>
+__global__ __launch_bounds__(kScanThreads) void evolve_scan_persistent_kernel(ScanArgs<T> a, int numItems) {
+    __shared__ int next;
+    if (*a.allDone) return;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = next;
+        __syncthreads();
+        if (item >= numItems) break;
+        scan_item<T, DIM, PPT>(a, a.items[item]);
+    }
+}
+
+// per-iteration reset of the work queue
+__global__ void iteration_prologue_kernel(int* __restrict__ queue, const int* __restrict__ allDone) {
+    if (*allDone) return;
+    *queue = 0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -348,19 +375,31 @@ __global__ __launch_bounds__(256) void fill_sentinel_kernel(T* __restrict__ S, i
 // ---------------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------------
+template <typename T, int DIM, int PPT>
+static void launch_scan_shape(const ScanArgs<T>& a, int numItems, int persistentBlocks, hipStream_t st) {
+    if (persistentBlocks > 0) {
+        hipLaunchKernelGGL(iteration_prologue_kernel, dim3(1), dim3(1), 0, st, a.queue, a.allDone);
+        hipLaunchKernelGGL((evolve_scan_persistent_kernel<T, DIM, PPT>), dim3(std::min(persistentBlocks, numItems)),
+                           dim3(kScanThreads), 0, st, a, numItems);
+    } else {
+        hipLaunchKernelGGL((evolve_scan_kernel<T, DIM, PPT>), dim3(numItems), dim3(kScanThreads), 0, st, a);
+    }
+}
+
 template <typename T>
-hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, hipStream_t st) {
+hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, int persistentBlocks,
+                              hipStream_t st) {
     if (numItems <= 0) return hipSuccess;
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
         if (ppt == 4) {
-            hipLaunchKernelGGL((evolve_scan_kernel<T, DIM, 4>), dim3(numItems), dim3(kScanThreads), 0, st, a);
+            launch_scan_shape<T, DIM, 4>(a, numItems, persistentBlocks, st);
             ok = true;
         } else if (ppt == 2) {
-            hipLaunchKernelGGL((evolve_scan_kernel<T, DIM, 2>), dim3(numItems), dim3(kScanThreads), 0, st, a);
+            launch_scan_shape<T, DIM, 2>(a, numItems, persistentBlocks, st);
             ok = true;
         } else if (ppt == 1) {
-            hipLaunchKernelGGL((evolve_scan_kernel<T, DIM, 1>), dim3(numItems), dim3(kScanThreads), 0, st, a);
+            launch_scan_shape<T, DIM, 1>(a, numItems, persistentBlocks, st);
             ok = true;
         }
     });
@@ -407,8 +446,8 @@ hipError_t launch_fill_sentinel(T* S, int64_t sStride, int dim, const SegDesc* s
     return hipGetLastError();
 }
 
-template hipError_t launch_evolve_scan<float>(const ScanArgs<float>&, int, int, int, hipStream_t);
-template hipError_t launch_evolve_scan<double>(const ScanArgs<double>&, int, int, int, hipStream_t);
+template hipError_t launch_evolve_scan<float>(const ScanArgs<float>&, int, int, int, int, hipStream_t);
+template hipError_t launch_evolve_scan<double>(const ScanArgs<double>&, int, int, int, int, hipStream_t);
 template hipError_t launch_chain_sum_shift<float>(const ShiftArgs<float>&, int, int64_t, hipStream_t);
 template hipError_t launch_chain_sum_shift<double>(const ShiftArgs<double>&, int, int64_t, hipStream_t);
 template hipError_t launch_fill_sentinel<float>(float*, int64_t, int, const SegDesc*, const int*, int, hipStream_t);

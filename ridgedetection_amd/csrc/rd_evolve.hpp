@@ -23,6 +23,7 @@ struct ScanArgs {
     const T* segRSqr;
     const int* allDone;
     int* labels;   // per sample: chosen slot, or -1 when outside every r-ball
+    int* queue;    // work-queue head of the persistent form (reset every iteration)
 };
 
 template <typename T>
@@ -50,8 +51,11 @@ struct ShiftArgs {
     unsigned short* outMoved;  // optional: per-slot bit mask of moved coordinates
 };
 
+// persistentBlocks > 0: that many persistent blocks pull the work items from a device
+// queue; 0: one block per work item.
 template <typename T>
-hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, hipStream_t st);
+hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, int persistentBlocks,
+                              hipStream_t st);
 // Gathers the members' coordinates in chain order, then sums and shifts; slots
 // [0, a.totalSlots) are visited (pass the live bound, not the capacity).
 template <typename T>
