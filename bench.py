@@ -68,6 +68,56 @@ def cpu_baseline(P, S, r, budget_s=20.0):
                       f"{el:.1f} s on {threads} host threads"}
 
 
+def batched_vs_looped(count=512, loop_every=8, seed=7):
+    """512 clouds (256 with d=2, 256 with d=3), sizes log-uniform in [2000, 500000], each
+    a noisy spiral / helix / segment chain with r1 = 4 sigma, r2 = 2 r1: batched (one
+    launch per phase for all clouds of a dimension) against a per-cloud loop over every
+    `loop_every`-th cloud.  Both forms start from host arrays (uploads included)."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context
+    log(f"bench: generating {count} clouds")
+    clouds = synth.batched_clouds(count=count, seed=seed)
+    total_pts = sum(len(p) for p, _, _ in clouds)
+    by_dim = {}
+    for i, (p, r1, r2) in enumerate(clouds):
+        by_dim.setdefault(p.shape[1], []).append(i)
+    ctxs = {d: Context(d, np.float32) for d in by_dim}
+    # warm-up (kernel loading, allocations) on a few small clouds of each dimension
+    for d, idx in by_dim.items():
+        small = sorted(idx, key=lambda i: len(clouds[i][0]))[:4]
+        ctxs[d].batch([clouds[i][0] for i in small], [clouds[i][1] for i in small], [clouds[i][2] for i in small])
+    log("bench: batched run")
+    t0 = time.perf_counter()
+    out_counts = 0
+    for d, idx in by_dim.items():
+        res = ctxs[d].batch([clouds[i][0] for i in idx], [clouds[i][1] for i in idx], [clouds[i][2] for i in idx])
+        out_counts += sum(len(r) for r in res)
+    t_batch = time.perf_counter() - t0
+    log(f"bench: batched {count} clouds in {t_batch:.2f} s")
+    sample = list(range(0, count, loop_every))
+    for d in ctxs:       # warm single-cloud paths
+        i = by_dim[d][0]
+        ctxs[d].set_points(clouds[i][0])
+        ctxs[d].ridge_detection(clouds[i][1], clouds[i][2])
+    t0 = time.perf_counter()
+    for i in sample:
+        p, r1, r2 = clouds[i]
+        c = ctxs[p.shape[1]]
+        c.set_points(p)
+        c.ridge_detection(r1, r2)
+    t_loop = time.perf_counter() - t0
+    log(f"bench: looped {len(sample)} clouds in {t_loop:.2f} s")
+    for c in ctxs.values():
+        c.close()
+    batched_rate = count / t_batch
+    loop_rate = len(sample) / t_loop
+    return {"clouds": count, "points": total_pts, "ridge_points": out_counts,
+            "batched_clouds_per_s": batched_rate, "looped_clouds_per_s": loop_rate,
+            "batched_over_looped": batched_rate / loop_rate,
+            "looped_sample": f"every {loop_every}th cloud ({len(sample)} clouds, same size distribution)",
+            "batched_s": t_batch, "looped_s": t_loop}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +127,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--ppt", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -152,11 +203,13 @@ def main():
                      "flops_per_launch": flops},
         "secondary": secondary,
     }
+    ctx.close()
+    if not args.no_batch:
+        result["secondary"]["batched_512_clouds"] = batched_vs_looped()
     if not args.no_cpu:
         log("bench: CPU baseline")
         result["cpu_baseline"] = cpu_baseline(P, S0, r)
     print(json.dumps(result), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

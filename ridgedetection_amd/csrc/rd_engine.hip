@@ -75,7 +75,7 @@ __global__ void relayout_slots_kernel(const T* __restrict__ S, int64_t sStride, 
 template <typename T>
 Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
     if (dim < 1 || dim > kMaxDim) throw std::invalid_argument("dimension must be in [1, 12]");
-    flags_.reserve(4);
+    flags_.reserve(8);   // [0] all done, [1] iterations, [2] overflow, [3] queue, [4] active items
     dN_.reserve(1);
     dM_.reserve(1);
 }
@@ -231,6 +231,28 @@ void Engine<T>::buildWorkItems() {
 }
 
 // Longest-first order for the persistent scan's work queue: cost ~ samples x live chosen.
+// Active work items (segments not done), in cost order, with their dense output offsets;
+// the count and the total live on the device so a captured graph stays valid.
+template <typename T>
+void Engine<T>::rebuildActive(const std::vector<int>& segDone) {
+    std::vector<WorkItem> act;
+    act.reserve(items_.size());
+    int64_t off = 0;
+    for (WorkItem it : items_) {
+        if (segDone[it.seg]) continue;
+        it.pad = (int32_t)off;
+        off += it.end - it.start;
+        act.push_back(it);
+    }
+    const int n = (int)act.size();
+    if (n > 0)
+        check(hipMemcpyAsync(dItems_.get(), act.data(), sizeof(WorkItem) * n, hipMemcpyHostToDevice, st_), "items");
+    check(hipMemcpyAsync(flags_.get() + 4, &n, sizeof(int), hipMemcpyHostToDevice, st_), "items");
+    check(hipMemcpyAsync(dN_.get(), &off, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
+    check(hipStreamSynchronize(st_), "rebuildActive");
+    activeWork_ = off;
+}
+
 template <typename T>
 void Engine<T>::orderItemsByCost() {
     if (items_.size() < 2) return;
@@ -264,6 +286,10 @@ void Engine<T>::ensureEvolveScratch() {
     labels_.reserve(totalP);
     keys_.reserve(totalP);
     members_.reserve(totalP);
+    if (segs_.size() > 1) {
+        cKeys_.reserve(totalP);
+        cPos_.reserve(totalP);
+    }
     covered_.reserve(totalP);
     const int64_t slots = std::max<int64_t>(totalSlots_, 1);
     chainOff_.reserve(slots);
@@ -409,7 +435,9 @@ void Engine<T>::choose() {
 template <typename T>
 void Engine<T>::enqueueIteration(bool record, int slot) {
     ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
-                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3};
+                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
+                   compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
+                   compact() ? flags_.get() + 4 : nullptr};
     const bool ev = options.profile && slot >= 0;
     if (ev) {
         while ((int)events_.size() < 3 * (slot + 1)) {
@@ -419,12 +447,16 @@ void Engine<T>::enqueueIteration(bool record, int slot) {
         }
         check(hipEventRecord(events_[3 * slot], st_), "event");
     }
-    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, persistentBlocks(), st_), "evolve_scan");
+    // one segment: a block per work item; batches of segments: persistent blocks on the
+    // active-item queue
+    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, compact() ? persistentBlocks() : 0, st_),
+          "evolve_scan");
     if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
     const int64_t totalP = totalP_;
     // labels are live slots < keyBound_ (counts only shrink between refreshes)
     const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    check(stable_sort_by_key(labels_.get(), nullptr, dN_.get(), totalP, (int)std::max<int64_t>(kb, 2),
+    check(stable_sort_by_key(compact() ? cKeys_.get() : labels_.get(), compact() ? cPos_.get() : nullptr,
+                             dN_.get(), totalP, (int)std::max<int64_t>(kb, 2),
                              keys_.get(), members_.get(), dM_.get(), sortWs_.get(), st_, flags_.get()),
           "sort");
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), kb, 0, flags_.get());
@@ -460,6 +492,7 @@ void Engine<T>::evolveFixed(long long iters) {
     int f[2] = {0, 0};
     check(hipMemcpyAsync(dSegDone_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    if (compact()) rebuildActive(zeros);
     const int batch = std::max(1, options.itersPerSync);
     long long done = 0;
     hipGraphExec_t exec = nullptr;
@@ -534,9 +567,11 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     check(hipMemcpyAsync(dSegIter_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "iter");
     check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    if (compact()) rebuildActive(done);
     const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
     long long iters = 0;
     const int batch = std::max(1, options.itersPerSync);
+    std::vector<int> doneNow(ns);
     // the iterations after convergence in a batch return at once (device all-done flag)
     hipGraphExec_t exec = nullptr;
     const bool useGraph = options.graph && !options.profile && captureIterations(batch, true, capIter, &exec);
@@ -559,6 +594,14 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         if (iters >= kEvolveGuard) {
             if (exec) (void)hipGraphExecDestroy(exec);
             throw std::runtime_error("evolve did not converge within the iteration guard");
+        }
+        if (compact()) {
+            // shrink the active work list once half of it belongs to converged segments
+            check(hipMemcpy(doneNow.data(), dSegDone_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "done");
+            int64_t remaining = 0;
+            for (const WorkItem& it : items_)
+                if (!doneNow[it.seg]) remaining += it.end - it.start;
+            if (2 * remaining <= activeWork_) rebuildActive(doneNow);
         }
     }
     if (exec) (void)hipGraphExecDestroy(exec);

@@ -188,6 +188,12 @@ private:
     void buildWorkItems();
     void orderItemsByCost();
     std::vector<WorkItem> items_;
+    // compact mode (more than one segment): the scan writes (label, position) pairs of the
+    // work items of active segments only, into a dense range the sort runs on
+    bool compact() const { return segs_.size() > 1; }
+    DevBuf<int> cKeys_, cPos_;
+    int64_t activeWork_ = 0;
+    void rebuildActive(const std::vector<int>& segDone);
     int persistentBlocks_ = 0;
     int persistentBlocks();
     // captures `batch` iterations (+ finalize when `finalize`) into an executable graph;

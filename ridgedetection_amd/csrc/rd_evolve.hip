@@ -59,8 +59,19 @@ __device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem i
     const SegDesc sd = a.segs[it.seg];
     const int tid = threadIdx.x;
     int* __restrict__ lab = a.labels + sd.pOff;
+    // compact output: slot of sample i is it.pad + (i - it.start)
+    int* __restrict__ ck = a.outKeys ? a.outKeys + it.pad - it.start : nullptr;
+    int* __restrict__ cp = a.outKeys ? a.outPos + it.pad - it.start : nullptr;
+    auto put = [&](int i, int label) {
+        if (ck) {
+            ck[i] = label;
+            cp[i] = (int)(sd.pOff + i);
+        } else {
+            lab[i] = label;
+        }
+    };
     if (a.segDone[it.seg]) {
-        for (int i = it.start + tid; i < it.end; i += kScanThreads) lab[i] = -1;
+        for (int i = it.start + tid; i < it.end; i += kScanThreads) put(i, -1);
         return;
     }
     const int K = a.segK[it.seg];
@@ -83,7 +94,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem i
     if (K <= 0) {
 #pragma unroll
         for (int j = 0; j < PPT; ++j)
-            if (idx[j] < it.end) lab[idx[j]] = -1;
+            if (idx[j] < it.end) put(idx[j], -1);
         return;
     }
 
@@ -178,7 +189,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem i
     }
 #pragma unroll
     for (int j = 0; j < PPT; ++j)
-        if (idx[j] < it.end) lab[idx[j]] = label[j];
+        if (idx[j] < it.end) put(idx[j], label[j]);
 }
 
 // one block per work item
@@ -194,6 +205,7 @@ template <typename T, int DIM, int PPT>
 __global__ __launch_bounds__(kScanThreads) void evolve_scan_persistent_kernel(ScanArgs<T> a, int numItems) {
     __shared__ int next;
     if (*a.allDone) return;
+    if (a.numItemsDev) numItems = *a.numItemsDev;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
         __syncthreads();

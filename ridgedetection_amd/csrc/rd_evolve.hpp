@@ -24,6 +24,12 @@ struct ScanArgs {
     const int* allDone;
     int* labels;   // per sample: chosen slot, or -1 when outside every r-ball
     int* queue;    // work-queue head of the persistent form (reset every iteration)
+    // compact form (batches of segments): item it writes (label, position) pairs to
+    // outKeys/outPos[it.pad + (i - it.start)] instead of labels[]; the persistent loop
+    // stops at *numItemsDev, so the active item list can shrink between graph replays
+    int* outKeys;
+    int* outPos;
+    const int* numItemsDev;
 };
 
 template <typename T>
