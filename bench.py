@@ -13,7 +13,8 @@ Also reported on the same JSON line:
                   duration measured by HIP events on the library's stream;
   cpu_baseline -- the reference CPU evolve pass (OpenMP build of the reference headers, or
                   the oracle's OpenMP restatement when that build is absent) on the host;
-  secondary    -- per-iteration device time and the other phases of the run.
+  secondary    -- per-iteration device time; the 512-cloud batched run against a per-cloud
+                  loop; forward + backward time of the differentiated evolve.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--dtype f32|f64] [--no-cpu]
 """
@@ -118,6 +119,43 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
             "batched_s": t_batch, "looped_s": t_loop}
 
 
+def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
+    """Differentiated evolve on the 200k-point helix (sigma 0.05, r 0.2): forward pass of
+    `iters` iterations keeping checkpoints every `every` iterations, then the backward pass
+    (recomputation of the earlier runs + the VJP kernels) for the helix loss gradient.
+    Times are the library's own (device buffers; the host staging of the gradient arrays is
+    excluded), against the forward and against plain evolve iterations (no checkpoints)."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context
+    P = synth.helix3d(n, seed=2024, dtype=np.float32)
+    ctx = Context(3, np.float32)
+    ctx.set_points(P)
+    ctx.choose(r)
+    ctx.evolve_fwd(r, iters, every)           # warm-up (allocations, kernel loading)
+    K = ctx.count()
+    _, G = synth.helix_loss(ctx.points())
+    G = G.astype(np.float32)
+    ctx.evolve_bwd(G)
+    fwd, bwd, plain = [], [], []
+    for _ in range(reps):
+        ctx.choose(r)
+        t = ctx.timers()
+        ctx.evolve_fwd(r, iters, every)
+        ctx.evolve_bwd(G)
+        t2 = ctx.timers()
+        fwd.append(t2["forward_ms"] - t["forward_ms"])
+        bwd.append(t2["backward_ms"] - t["backward_ms"])
+        ctx.choose(r)
+        t0 = time.perf_counter()
+        ctx.evolve_fixed(r, iters)
+        plain.append((time.perf_counter() - t0) * 1e3)
+    ctx.close()
+    f, b, p = (float(np.median(x)) for x in (fwd, bwd, plain))
+    return {"points": n, "chosen": K, "iterations": iters, "checkpoint_every": every,
+            "forward_ms": f, "backward_ms": b, "plain_evolve_ms": p,
+            "fwd_plus_bwd_over_fwd": (f + b) / f, "fwd_plus_bwd_over_plain_evolve": (f + b) / p}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +166,7 @@ def main():
     ap.add_argument("--ppt", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
+    ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -150,16 +189,10 @@ def main():
     ctx.set_option(OPT_PPT, args.ppt)
     ctx.set_option(OPT_PROFILE, 1)
     ctx.evolve_fixed(r, args.warmup)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
-    sync()
+    # every library call returns with its stream synchronised
     t_before = ctx.timers()
     t0 = time.perf_counter()
     ctx.evolve_fixed(r, args.steps)
-    sync()
     el = time.perf_counter() - t0
     t_after = ctx.timers()
     log(f"bench: {args.steps} iterations in {el * 1e3:.2f} ms")
@@ -204,6 +237,9 @@ def main():
         "secondary": secondary,
     }
     ctx.close()
+    if not args.no_vjp and dt == np.float32:
+        log("bench: differentiated evolve (200k helix, 20 iterations)")
+        result["secondary"]["vjp_helix_200k"] = vjp_helix()
     if not args.no_batch:
         result["secondary"]["batched_512_clouds"] = batched_vs_looped()
     if not args.no_cpu:

@@ -84,6 +84,8 @@ typedef struct rdhip_timers {
     double scan_kernel_ms;   /* HIP-event time of the evolve scan kernel (RDHIP_OPT_PROFILE) */
     int64_t scan_launches;
     double iteration_ms;     /* HIP-event time of whole evolve iterations (RDHIP_OPT_PROFILE) */
+    double forward_ms;       /* rdhip_evolve_fwd (wall time, stream synchronised) */
+    double backward_ms;      /* rdhip_evolve_bwd without the staging of host buffers */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */
@@ -176,6 +178,29 @@ int rdhip_evolve_fixed(rdhip_ctx* ctx, double r, int64_t iters);
  * contFlag. */
 int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* counts, void* sums,
                            uint16_t* moved, int* out_flag);
+
+/* ---- differentiated evolve (no reference counterpart: the reference has no derivative) ----
+ *
+ * rdhip_evolve_fwd runs exactly `iters` evolve iterations with radius r from the current
+ * chosen set (as rdhip_evolve_fixed) and keeps what rdhip_evolve_bwd needs: the chosen set
+ * at the start of every run of `checkpoint_every` iterations, and the memberships of the
+ * last run.  checkpoint_every = 0 keeps the memberships of every iteration (iters x n
+ * int32 of device memory); otherwise earlier runs are recomputed from their checkpoints
+ * during the backward pass (about one more forward pass of work).
+ *
+ * rdhip_evolve_bwd is the vector-Jacobian product of that forward pass with the
+ * memberships (labels, counts and moved coordinates of every iteration) held fixed:
+ *   grad_chosen (in):  dL/dS_T, K x dim (the K chosen points after the forward pass);
+ *   grad_points (out): dL/dP, n x dim;
+ *   grad_seeds (out, optional): dL/dS_0, K x dim;
+ *   seed_to_points != 0: S_0 came from rdhip_choose, so dL/dS_0 is also added to the
+ *     samples it was drawn from (an error when the chosen set was set explicitly).
+ * Buffers are host or device (on_device) arrays in ROW_MAJOR or SOA layout (dense).  The
+ * chosen set is left as the forward pass produced it.  Any call that changes the points or
+ * the chosen set in between invalidates the forward pass (RDHIP_ERR_STATE). */
+int rdhip_evolve_fwd(rdhip_ctx* ctx, double r, int64_t iters, int64_t checkpoint_every);
+int rdhip_evolve_bwd(rdhip_ctx* ctx, const void* grad_chosen, void* grad_points, void* grad_seeds, int layout,
+                     int on_device, int seed_to_points);
 
 /* Decimate the chosen set with radius r2 and `rule`; keep (host, optional, capacity K)
  * receives 1 for kept and 0 for removed points, in the pre-decimation order. */

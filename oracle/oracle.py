@@ -123,6 +123,49 @@ class Oracle:
                                               self._scalar(P.dtype, r))
         return S, int(it)
 
+    # -- differentiated evolve (frozen memberships; restatement only) ---------------
+    def evolve_tape(self, P, S, r, iters):
+        """`iters` evolve iterations from S recording what the frozen map needs.
+        Returns (S_T, labels [iters, n] int32, counts [iters, K] int32, moved [iters, K, d] uint8)."""
+        P = np.ascontiguousarray(P)
+        S = np.array(S, dtype=P.dtype, copy=True, order="C")
+        n, d = P.shape
+        K = S.shape[0]
+        labels = np.empty((iters, n), dtype=np.int32)
+        counts = np.empty((iters, K), dtype=np.int32)
+        moved = np.empty((iters, K, d), dtype=np.uint8)
+        self._check(self._fn("evolve_tape", P.dtype)(_p(P), C.c_long(n), _p(S), C.c_long(K), C.c_int(d),
+                                                      self._scalar(P.dtype, r), C.c_long(iters), _p(labels),
+                                                      _p(counts), _p(moved)), "evolve_tape")
+        return S, labels, counts, moved
+
+    def evolve_replay(self, P, S0, labels, moved):
+        """The frozen-membership evolve map evaluated at (P, S0): returns S_T."""
+        P = np.ascontiguousarray(P)
+        S = np.array(S0, dtype=P.dtype, copy=True, order="C")
+        n, d = P.shape
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        moved = np.ascontiguousarray(moved, dtype=np.uint8)
+        self._check(self._fn("evolve_replay", P.dtype)(_p(P), C.c_long(n), _p(S), C.c_long(S.shape[0]), C.c_int(d),
+                                                        C.c_long(labels.shape[0]), _p(labels), _p(moved)),
+                    "evolve_replay")
+        return S
+
+    def evolve_vjp(self, labels, counts, moved, G, dtype=np.float64):
+        """Reverse mode of the frozen map: returns (dL/dP [n, d], dL/dS_0 [K, d]) for
+        G = dL/dS_T [K, d], in `dtype`."""
+        labels = np.ascontiguousarray(labels, dtype=np.int32)
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        moved = np.ascontiguousarray(moved, dtype=np.uint8)
+        iters, n = labels.shape
+        K, d = G.shape
+        g = np.array(G, dtype=dtype, copy=True, order="C")
+        gp = np.empty((n, d), dtype=dtype)
+        self._check(self._fn("evolve_vjp", dtype)(C.c_long(n), C.c_long(K), C.c_int(d), C.c_long(iters),
+                                                   _p(labels), _p(counts), _p(moved), _p(g), _p(gp)),
+                    "evolve_vjp")
+        return gp, g
+
     def decimate(self, S, r2):
         S = np.array(S, copy=True, order="C")
         K, d = S.shape

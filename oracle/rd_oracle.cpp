@@ -668,6 +668,75 @@ static long tiledRD(int alg, const T* P, long n, int dim, T r1, T r2, size_t cap
     return ns;
 }
 
+// ---------------------------------------------------------------------------------
+// Differentiated evolve with frozen memberships.  The reference has no derivative; these
+// three functions define the differentiable map the GPU forward/backward pair computes.
+// With the labels l_t(n) and the moved flags m_t(k,d) of a recorded run held fixed,
+//     S_{t+1}(k,d) = m_t(k,d) * (sum_{n: l_t(n)=k} P(n,d)) / c_t(k) + (1 - m_t(k,d)) * S_t(k,d),
+// which is exactly what a5 (shiftTowardMassCenter) does for that run.
+// ---------------------------------------------------------------------------------
+
+// `iters` reference iterations (a4 + a5) from S, recording labels [iters][n],
+// counts [iters][K] and moved flags [iters][K][dim].
+template <typename T>
+static void evolveTape(const T* P, long n, T* S, long K, int dim, T r, long iters, int* labels,
+                       int* counts, unsigned char* moved) {
+    std::vector<T> sums((size_t)std::max<long>(K, 1) * dim);
+    std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
+    for (long t = 0; t < iters; ++t)
+        evolveIteration<T>(P, blocks, S, K, dim, r, sums.data(), counts + t * K, labels + t * n,
+                           moved + t * K * dim);
+}
+
+// The frozen map above, evaluated at (P, S): members summed in sample order, as a4.
+template <typename T>
+static void evolveReplay(const T* P, long n, T* S, long K, int dim, long iters, const int* labels,
+                         const unsigned char* moved) {
+    std::vector<T> sums((size_t)std::max<long>(K, 1) * dim);
+    std::vector<int> counts((size_t)std::max<long>(K, 1));
+    for (long t = 0; t < iters; ++t) {
+        std::fill(sums.begin(), sums.end(), T(0));
+        std::fill(counts.begin(), counts.end(), 0);
+        const int* lab = labels + t * n;
+        for (long i = 0; i < n; ++i) {
+            if (lab[i] < 0) continue;
+            counts[lab[i]]++;
+            for (int d = 0; d < dim; ++d) sums[(long)lab[i] * dim + d] += P[i * dim + d];
+        }
+        const unsigned char* mv = moved + t * K * dim;
+        for (long k = 0; k < K; ++k)
+            for (int d = 0; d < dim; ++d)
+                if (mv[k * dim + d] && counts[k]) S[k * dim + d] = sums[k * dim + d] / (T)counts[k];
+    }
+}
+
+// Reverse mode of the frozen map.  G holds dL/dS_T on entry and dL/dS_0 on return;
+// gradP (n x dim) receives dL/dP through the member sums of every iteration.  S_0 itself
+// is an input here (when it was chosen from P, the caller adds G to those samples).
+template <typename T>
+static void evolveVjp(long n, long K, int dim, long iters, const int* labels, const int* counts,
+                      const unsigned char* moved, T* G, T* gradP) {
+    std::fill(gradP, gradP + n * dim, T(0));
+    std::vector<T> A((size_t)std::max<long>(K, 1) * dim);
+    for (long t = iters - 1; t >= 0; --t) {
+        const int* cnt = counts + t * K;
+        const unsigned char* mv = moved + t * K * dim;
+        for (long k = 0; k < K; ++k)
+            for (int d = 0; d < dim; ++d) {
+                const long j = k * dim + d;
+                A[j] = 0;
+                if (mv[j] && cnt[k]) {   // S_{t+1} = sum / c: d/dsum = 1/c, d/dS_t = 0
+                    A[j] = G[j] / (T)cnt[k];
+                    G[j] = 0;
+                }
+            }
+        const int* lab = labels + t * n;
+        for (long i = 0; i < n; ++i)
+            if (lab[i] >= 0)
+                for (int d = 0; d < dim; ++d) gradP[i * dim + d] += A[(long)lab[i] * dim + d];
+    }
+}
+
 }  // namespace oracle
 
 // =================================================================================
@@ -850,6 +919,27 @@ int oracle_num_threads() {
     long oracle_tiled_##SUF(int alg, const T* P, long n, int dim, T r1, T r2, long cap,              \
                             const long* tpd, int refinement, long maxEvolveIter, T* S, long Scap) {  \
         return tiledC<T>(alg, P, n, dim, r1, r2, cap, tpd, refinement, maxEvolveIter, S, Scap);      \
+    }                                                                                                \
+    long oracle_evolve_tape_##SUF(const T* P, long n, T* S, long K, int dim, T r, long iters,       \
+                                  int* labels, int* counts, unsigned char* moved) {                  \
+        return guarded([&]() -> long {                                                               \
+            oracle::evolveTape<T>(P, n, S, K, dim, r, iters, labels, counts, moved);                 \
+            return 0;                                                                                \
+        });                                                                                          \
+    }                                                                                                \
+    long oracle_evolve_replay_##SUF(const T* P, long n, T* S, long K, int dim, long iters,          \
+                                    const int* labels, const unsigned char* moved) {                 \
+        return guarded([&]() -> long {                                                               \
+            oracle::evolveReplay<T>(P, n, S, K, dim, iters, labels, moved);                          \
+            return 0;                                                                                \
+        });                                                                                          \
+    }                                                                                                \
+    long oracle_evolve_vjp_##SUF(long n, long K, int dim, long iters, const int* labels,            \
+                                 const int* counts, const unsigned char* moved, T* G, T* gradP) {    \
+        return guarded([&]() -> long {                                                               \
+            oracle::evolveVjp<T>(n, K, dim, iters, labels, counts, moved, G, gradP);                 \
+            return 0;                                                                                \
+        });                                                                                          \
     }                                                                                                \
     long oracle_tree_##SUF(int kind, const T* P, long n, int dim, T r1, long cap, const long* tpd,   \
                            long* leafInfo, long infoCap, long* ownIdx, long ownCap, long* nbIdx,     \

@@ -30,7 +30,7 @@ EXPORTS = [
     "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
-    "rdhip_batch_ridge_detection",
+    "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH = 0, 1, 2, 3, 4
@@ -40,7 +40,8 @@ class Timers(C.Structure):
     _fields_ = [("whole_ms", C.c_double), ("build_ms", C.c_double), ("choose_ms", C.c_double),
                 ("evolve_ms", C.c_double), ("decimate_ms", C.c_double), ("refine_ms", C.c_double),
                 ("evolve_iterations", C.c_int64), ("rounds", C.c_int64),
-                ("scan_kernel_ms", C.c_double), ("scan_launches", C.c_int64), ("iteration_ms", C.c_double)]
+                ("scan_kernel_ms", C.c_double), ("scan_launches", C.c_int64), ("iteration_ms", C.c_double),
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
 
 
 _lib = None
@@ -76,6 +77,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_build_tree": (i32, [vp, i32, C.c_double, i64, vp, C.POINTER(i64)]),
         "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
         "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
+        "rdhip_evolve_fwd": (i32, [vp, C.c_double, i64, i64]),
+        "rdhip_evolve_bwd": (i32, [vp, vp, vp, vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -234,6 +237,27 @@ class Context:
 
     def evolve_fixed(self, r, iters):
         self._check(self.lib.rdhip_evolve_fixed(self.h, float(r), int(iters)))
+
+    def evolve_fwd(self, r, iters, checkpoint_every=0):
+        """Exactly `iters` evolve iterations, keeping what evolve_bwd needs."""
+        self._check(self.lib.rdhip_evolve_fwd(self.h, float(r), int(iters), int(checkpoint_every)))
+
+    def evolve_bwd(self, grad_chosen, seed_to_points=True):
+        """VJP of the last evolve_fwd with memberships frozen: grad_chosen = dL/dS_T (K x dim,
+        host).  Returns (dL/dP [n x dim], dL/dS_0 [K x dim])."""
+        g = np.ascontiguousarray(grad_chosen, dtype=self.dtype)
+        gp = np.empty((self.n, self.dim), dtype=self.dtype)
+        gs = np.empty_like(g)
+        self._check(self.lib.rdhip_evolve_bwd(self.h, _ptr(g), _ptr(gp), _ptr(gs), ROW_MAJOR, 0,
+                                              1 if seed_to_points else 0))
+        return gp, gs
+
+    def evolve_bwd_device(self, grad_chosen_ptr: int, grad_points_ptr: int, grad_seeds_ptr: int = 0,
+                          seed_to_points=True):
+        """Device form (dense SoA buffers: dim x K in, dim x n out)."""
+        self._check(self.lib.rdhip_evolve_bwd(self.h, C.c_void_p(grad_chosen_ptr), C.c_void_p(grad_points_ptr),
+                                              C.c_void_p(grad_seeds_ptr) if grad_seeds_ptr else None, SOA, 1,
+                                              1 if seed_to_points else 0))
 
     def evolve_iteration(self, r):
         K = self.count()

@@ -4,6 +4,7 @@
 #include "rd_tiled.hpp"
 #include "rd_tree.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -47,6 +48,8 @@ int guarded(F&& f) {
         return fail(RDHIP_ERR_HIP, e.what());
     } catch (const std::invalid_argument& e) {
         return fail(RDHIP_ERR_INVALID, e.what());
+    } catch (const std::logic_error& e) {
+        return fail(RDHIP_ERR_STATE, e.what());
     } catch (const std::bad_alloc&) {
         return fail(RDHIP_ERR_RUNTIME, "host allocation failed");
     } catch (const std::exception& e) {
@@ -263,6 +266,8 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->scan_kernel_ms = e.timers.scanKernelMs;
         out->scan_launches = e.timers.scanLaunches;
         out->iteration_ms = e.timers.iterationMs;
+        out->forward_ms = e.timers.forwardMs;
+        out->backward_ms = e.timers.backwardMs;
         return RDHIP_OK;
     });
 }
@@ -448,6 +453,65 @@ int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* c
         if (moved)
             for (int64_t k = 0; k < K; ++k) moved[k] = mv[k];
         if (out_flag) *out_flag = flag;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_evolve_fwd(rdhip_ctx* ctx, double r, int64_t iters, int64_t checkpoint_every) {
+    if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    if (iters < 1) return fail(RDHIP_ERR_INVALID, "iteration count must be positive");
+    if (checkpoint_every < 0) return fail(RDHIP_ERR_INVALID, "negative checkpoint interval");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points (call rdhip_choose first)");
+        e.setRadii({(T)r}, {(T)r});
+        e.evolveForward(iters, (int)std::min<int64_t>(checkpoint_every, iters));
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_evolve_bwd(rdhip_ctx* ctx, const void* grad_chosen, void* grad_points, void* grad_seeds, int layout,
+                     int on_device, int seed_to_points) {
+    if (!grad_chosen || !grad_points) return fail(RDHIP_ERR_INVALID, "null gradient buffer");
+    if (layout != RDHIP_ROW_MAJOR && layout != RDHIP_SOA && layout != RDHIP_COL_MAJOR)
+        return fail(RDHIP_ERR_INVALID, "unknown layout");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        const int dim = ctx->dim;
+        const int64_t K = e.totalCount();
+        const int64_t n = e.numPoints();
+        const bool soa = layout != RDHIP_ROW_MAJOR;
+        if (on_device && soa) {
+            e.evolveBackward(static_cast<const T*>(grad_chosen), std::max<int64_t>(K, 1), static_cast<T*>(grad_points),
+                             std::max<int64_t>(n, 1), static_cast<T*>(grad_seeds), std::max<int64_t>(K, 1),
+                             seed_to_points != 0);
+            return RDHIP_OK;
+        }
+        // staged: dense SoA device buffers, converted on the host
+        const hipMemcpyKind up = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+        std::vector<T> g((size_t)K * dim), gHost((size_t)K * dim);
+        if (K > 0) check(hipMemcpy(gHost.data(), grad_chosen, sizeof(T) * K * dim, up), "gradient");
+        for (int64_t k = 0; k < K; ++k)
+            for (int d = 0; d < dim; ++d) g[(size_t)d * K + k] = soa ? gHost[(size_t)d * K + k] : gHost[(size_t)k * dim + d];
+        DevBuf<T> dG, dP, dS0;
+        dG.reserve((size_t)std::max<int64_t>(K, 1) * dim);
+        dP.reserve((size_t)std::max<int64_t>(n, 1) * dim);
+        if (grad_seeds) dS0.reserve((size_t)std::max<int64_t>(K, 1) * dim);
+        if (K > 0) check(hipMemcpy(dG.get(), g.data(), sizeof(T) * K * dim, hipMemcpyHostToDevice), "gradient");
+        e.evolveBackward(dG.get(), std::max<int64_t>(K, 1), dP.get(), std::max<int64_t>(n, 1),
+                         grad_seeds ? dS0.get() : nullptr, std::max<int64_t>(K, 1), seed_to_points != 0);
+        auto down = [&](T* dst, const T* src, int64_t cnt) {
+            std::vector<T> h((size_t)cnt * dim), o((size_t)cnt * dim);
+            if (cnt > 0) check(hipMemcpy(h.data(), src, sizeof(T) * cnt * dim, hipMemcpyDeviceToHost), "gradient");
+            for (int64_t i = 0; i < cnt; ++i)
+                for (int d = 0; d < dim; ++d)
+                    o[soa ? (size_t)d * cnt + i : (size_t)i * dim + d] = h[(size_t)d * cnt + i];
+            if (cnt > 0)
+                check(hipMemcpy(dst, o.data(), sizeof(T) * cnt * dim, on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost),
+                      "gradient");
+        };
+        down(static_cast<T*>(grad_points), dP.get(), n);
+        if (grad_seeds) down(static_cast<T*>(grad_seeds), dS0.get(), K);
         return RDHIP_OK;
     });
 }

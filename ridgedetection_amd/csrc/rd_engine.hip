@@ -90,6 +90,7 @@ Engine<T>::~Engine() {
 template <typename T>
 void Engine<T>::setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice) {
     if (n < 0) throw std::invalid_argument("negative point count");
+    ++version_;
     nPoints_ = n;
     pStride_ = round_up(std::max<int64_t>(n, 1), 64);
     P_.reserve((size_t)pStride_ * dim_);
@@ -135,6 +136,7 @@ void Engine<T>::setSegments(const std::vector<SegDesc>& segs, const std::vector<
     useIndex_ = indexed;
     if (capacities.size() != segs.size() || r1.size() != segs.size() || r2.size() != segs.size())
         throw std::invalid_argument("segment arrays differ in length");
+    ++version_;
     segs_ = segs;
     r1_ = r1;
     r2_ = r2;
@@ -317,6 +319,7 @@ template <typename T>
 void Engine<T>::compactSlots() {
     const int ns = (int)segs_.size();
     if (ns == 0) return;
+    ++version_;
     std::vector<int> K = countsHost();
     std::vector<int64_t> oldOff(ns), newOff(ns);
     std::vector<int> copyCnt(ns);
@@ -379,6 +382,8 @@ template <typename T>
 void Engine<T>::choose() {
     const int ns = (int)segs_.size();
     if (ns == 0) return;
+    ++version_;
+    chosenFromPoints_ = true;
     auto t0 = std::chrono::steady_clock::now();
     constexpr int kLarge = 32768;
     std::vector<int> large(ns);
@@ -433,9 +438,10 @@ void Engine<T>::choose() {
 // evolve
 // ------------------------------------------------------------------------------------
 template <typename T>
-void Engine<T>::enqueueIteration(bool record, int slot) {
+void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
+    int* labels = log ? log->labels : labels_.get();
     ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
-                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
+                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
                    compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
                    compact() ? flags_.get() + 4 : nullptr};
     const bool ev = options.profile && slot >= 0;
@@ -452,10 +458,12 @@ void Engine<T>::enqueueIteration(bool record, int slot) {
     check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, compact() ? persistentBlocks() : 0, st_),
           "evolve_scan");
     if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
-    const int64_t totalP = totalP_;
+    // positions the sort and the member passes cover: the dense active range of a batch
+    // (it only shrinks; graphs are captured again when it does), else every position
+    const int64_t totalP = compact() ? std::max<int64_t>(activeWork_, 1) : totalP_;
     // labels are live slots < keyBound_ (counts only shrink between refreshes)
     const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    check(stable_sort_by_key(compact() ? cKeys_.get() : labels_.get(), compact() ? cPos_.get() : nullptr,
+    check(stable_sort_by_key(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr,
                              dN_.get(), totalP, (int)std::max<int64_t>(kb, 2),
                              keys_.get(), members_.get(), dM_.get(), sortWs_.get(), st_, flags_.get()),
           "sort");
@@ -465,8 +473,9 @@ void Engine<T>::enqueueIteration(bool record, int slot) {
           "chain_bounds");
     ShiftArgs<T> ha{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
-                    totalP, chainOff_.get(), chainEnd_.get(), flags_.get(), record ? counts_.get() : nullptr,
-                    record ? sums_.get() : nullptr, record ? moved_.get() : nullptr};
+                    totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
+                    log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
+                    log ? log->moved : (record ? moved_.get() : nullptr)};
     check(launch_chain_sum_shift<T>(ha, dim_, totalP, st_), "chain_sum_shift");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
 }
@@ -487,6 +496,7 @@ template <typename T>
 void Engine<T>::evolveFixed(long long iters) {
     const int ns = (int)segs_.size();
     if (ns == 0 || iters <= 0) return;
+    ++version_;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<int> zeros(ns, 0);
     int f[2] = {0, 0};
@@ -554,6 +564,7 @@ template <typename T>
 long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) {
     const int ns = (int)segs_.size();
     if (ns == 0) return 0;
+    ++version_;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<int> done(ns, 0), zeros(ns, 0);
     int anyActive = 0;
@@ -574,7 +585,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     std::vector<int> doneNow(ns);
     // the iterations after convergence in a batch return at once (device all-done flag)
     hipGraphExec_t exec = nullptr;
-    const bool useGraph = options.graph && !options.profile && captureIterations(batch, true, capIter, &exec);
+    bool useGraph = options.graph && !options.profile && captureIterations(batch, true, capIter, &exec);
     for (;;) {
         if (useGraph) {
             check(hipGraphLaunch(exec, st_), "hipGraphLaunch");
@@ -601,7 +612,14 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
             int64_t remaining = 0;
             for (const WorkItem& it : items_)
                 if (!doneNow[it.seg]) remaining += it.end - it.start;
-            if (2 * remaining <= activeWork_) rebuildActive(doneNow);
+            if (2 * remaining <= activeWork_) {
+                rebuildActive(doneNow);
+                if (exec) {
+                    (void)hipGraphExecDestroy(exec);
+                    exec = nullptr;
+                    useGraph = captureIterations(batch, true, capIter, &exec);
+                }
+            }
         }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -614,6 +632,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
 template <typename T>
 int Engine<T>::evolveIteration(std::vector<int>* labels, std::vector<int>* counts, std::vector<T>* sums,
                                std::vector<unsigned short>* moved) {
+    ++version_;
     const int ns = (int)segs_.size();
     std::vector<int> zeros(ns, 0);
     int f[2] = {0, 0};
@@ -654,12 +673,117 @@ int Engine<T>::evolveIteration(std::vector<int>* labels, std::vector<int>* count
 }
 
 // ------------------------------------------------------------------------------------
+// differentiated evolve (memberships frozen)
+// ------------------------------------------------------------------------------------
+template <typename T>
+void Engine<T>::recordRun(int run) {
+    const long long first = (long long)run * tape_.every;
+    const int steps = (int)std::min<long long>(tape_.every, tape_.iters - first);
+    const int64_t kc = std::max(tape_.K, 1);
+    for (int u = 0; u < steps; ++u) {
+        IterLog lg{tape_.labels.get() + (int64_t)u * tape_.positions, tape_.counts.get() + (int64_t)u * kc,
+                   tape_.moved.get() + (int64_t)u * kc};
+        enqueueIteration(false, -1, &lg);
+    }
+}
+
+template <typename T>
+void Engine<T>::evolveForward(long long iters, int every) {
+    if (segs_.size() != 1 || useIndex_)
+        throw std::invalid_argument("the evolve forward/backward pair runs on one whole cloud");
+    if (iters <= 0 || iters > (1LL << 24)) throw std::invalid_argument("iteration count must be in [1, 2^24]");
+    ++version_;
+    auto t0 = std::chrono::steady_clock::now();
+    tape_.valid = false;
+    if (every <= 0 || every > iters) every = (int)iters;
+    tape_.iters = iters;
+    tape_.every = every;
+    tape_.runs = (int)((iters + every - 1) / every);
+    tape_.K = countsHost()[0];
+    tape_.positions = totalP_;
+    tape_.r = r1_[0];
+    const int64_t kc = std::max(tape_.K, 1);
+    const size_t sBytes = sizeof(T) * (size_t)sStride_ * dim_;
+    tape_.ckpt.reserve((size_t)tape_.runs * sStride_ * dim_);
+    tape_.finalS.reserve((size_t)sStride_ * dim_);
+    tape_.labels.reserve((size_t)every * tape_.positions);
+    tape_.counts.reserve((size_t)every * kc);
+    tape_.moved.reserve((size_t)every * kc);
+    int zero = 0, f[2] = {0, 0};
+    check(hipMemcpyAsync(dSegDone_.get(), &zero, sizeof(int), hipMemcpyHostToDevice, st_), "done");
+    check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    for (int run = 0; run < tape_.runs; ++run) {
+        check(hipMemcpyAsync(tape_.ckpt.get() + (size_t)run * sStride_ * dim_, S_.get(), sBytes,
+                             hipMemcpyDeviceToDevice, st_),
+              "checkpoint");
+        recordRun(run);
+    }
+    check(hipMemcpyAsync(tape_.finalS.get(), S_.get(), sBytes, hipMemcpyDeviceToDevice, st_), "checkpoint");
+    check(hipStreamSynchronize(st_), "evolveForward");
+    tape_.loggedRun = tape_.runs - 1;
+    tape_.version = version_;
+    tape_.valid = true;
+    timers.evolveIterations += iters;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    timers.evolveMs += ms;
+    timers.forwardMs += ms;
+}
+
+template <typename T>
+void Engine<T>::evolveBackward(const T* gS, int64_t gsStride, T* gP, int64_t gpStride, T* gS0, int64_t gs0Stride,
+                               bool seedToPoints) {
+    if (!tape_.valid || tape_.version != version_)
+        throw std::logic_error("no evolve forward pass to differentiate (the chosen set changed or none ran)");
+    if (seedToPoints && !chosenFromPoints_)
+        throw std::invalid_argument("the chosen set was not drawn from the points (set explicitly)");
+    auto t0 = std::chrono::steady_clock::now();
+    const int K = tape_.K;
+    const int64_t kc = std::max(K, 1);
+    tape_.G.reserve((size_t)dim_ * kc);
+    tape_.A.reserve((size_t)tape_.every * dim_ * kc);
+    if (K > 0)
+        check(hipMemcpy2DAsync(tape_.G.get(), sizeof(T) * kc, gS, sizeof(T) * gsStride, sizeof(T) * K, dim_,
+                               hipMemcpyDeviceToDevice, st_),
+              "upstream");
+    // recomputed runs replay the forward's iterations from their checkpoints
+    const T r1sq = tape_.r * tape_.r;
+    check(hipMemcpyAsync(dSegR1Sqr_.get(), &r1sq, sizeof(T), hipMemcpyHostToDevice, st_), "r1");
+    int zero = 0, f[2] = {0, 0};
+    check(hipMemcpyAsync(dSegDone_.get(), &zero, sizeof(int), hipMemcpyHostToDevice, st_), "done");
+    check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    const size_t sBytes = sizeof(T) * (size_t)sStride_ * dim_;
+    for (int run = tape_.runs - 1; run >= 0; --run) {
+        if (run != tape_.loggedRun) {
+            check(hipMemcpyAsync(S_.get(), tape_.ckpt.get() + (size_t)run * sStride_ * dim_, sBytes,
+                                 hipMemcpyDeviceToDevice, st_),
+                  "restore");
+            recordRun(run);
+            tape_.loggedRun = run;
+        }
+        const long long first = (long long)run * tape_.every;
+        VjpArgs<T> a{tape_.labels.get(), tape_.positions, tape_.counts.get(), tape_.moved.get(), kc,
+                     (int)std::min<long long>(tape_.every, tape_.iters - first), K, tape_.G.get(), kc,
+                     tape_.A.get(), kc, gP, gpStride, nPoints_, run != tape_.runs - 1 ? 1 : 0};
+        check(launch_vjp_run<T>(a, dim_, st_), "vjp");
+    }
+    if (seedToPoints) check(launch_vjp_seed<T>(tape_.G.get(), kc, chosenIdx_.get(), K, dim_, gP, gpStride, st_), "seed");
+    if (gS0 && K > 0)
+        check(hipMemcpy2DAsync(gS0, sizeof(T) * gs0Stride, tape_.G.get(), sizeof(T) * kc, sizeof(T) * K, dim_,
+                               hipMemcpyDeviceToDevice, st_),
+              "seed gradient");
+    check(hipMemcpyAsync(S_.get(), tape_.finalS.get(), sBytes, hipMemcpyDeviceToDevice, st_), "restore");
+    check(hipStreamSynchronize(st_), "evolveBackward");
+    timers.backwardMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------------------------
 // decimate
 // ------------------------------------------------------------------------------------
 template <typename T>
 void Engine<T>::decimate(bool noCopyRule, std::vector<unsigned char>* keep) {
     const int ns = (int)segs_.size();
     if (ns == 0) return;
+    ++version_;
     auto t0 = std::chrono::steady_clock::now();
     const int ng = (int)groupBegin_.size();
     const int64_t slots = std::max<int64_t>(totalSlots_, 1);
@@ -829,6 +953,8 @@ std::vector<int> Engine<T>::chosenIndicesHost(int seg) {
 
 template <typename T>
 void Engine<T>::setChosen(int seg, const T* S, int64_t K, int layout) {
+    ++version_;
+    chosenFromPoints_ = false;
     if (seg < 0 || seg >= (int)segs_.size()) throw std::invalid_argument("segment out of range");
     if (K < 0 || K > segs_[seg].sCap) throw std::invalid_argument("too many chosen points for the segment");
     std::vector<T> soa((size_t)K * dim_);

@@ -7,6 +7,7 @@
 #include "rd_decimate.hpp"
 #include "rd_evolve.hpp"
 #include "rd_sort.hpp"
+#include "rd_vjp.hpp"
 
 #include <chrono>
 #include <stdexcept>
@@ -62,6 +63,8 @@ struct Timers {
     double scanKernelMs = 0;      // HIP-event time of the evolve scan kernel (when profiling)
     long long scanLaunches = 0;
     double iterationMs = 0;       // HIP-event time of whole evolve iterations (when profiling)
+    double forwardMs = 0;         // evolveForward calls (stream synchronised at the end)
+    double backwardMs = 0;        // evolveBackward calls, device buffers in and out
 };
 
 struct RunOptions {
@@ -125,6 +128,21 @@ public:
     int evolveIteration(std::vector<int>* labels, std::vector<int>* counts, std::vector<T>* sums,
                         std::vector<unsigned short>* moved);
     void decimate(bool noCopyRule, std::vector<unsigned char>* keep = nullptr);
+
+    // -- differentiated evolve (one segment) ------------------------------------------
+    // Forward: exactly `iters` iterations (as evolveFixed), keeping what the backward needs:
+    // the chosen set at the start of every run of `every` iterations (checkpoints) and the
+    // labels / counts / moved flags of the last run.  every <= 0 or >= iters keeps the logs
+    // of all iterations (no recomputation).
+    void evolveForward(long long iters, int every);
+    // Backward of the last forward with memberships frozen: gS = dL/dS_T (SoA, stride
+    // gsStride, K columns), gP = dL/dP (SoA, stride gpStride, n columns, overwritten),
+    // gS0 = dL/dS_0 (optional, SoA, stride gs0Stride).  Earlier runs are recomputed from
+    // their checkpoints.  seedToPoints: S_0 came from choose, so dL/dS_0 is added to the
+    // samples it was drawn from.  All pointers are device pointers.
+    void evolveBackward(const T* gS, int64_t gsStride, T* gP, int64_t gpStride, T* gS0, int64_t gs0Stride,
+                        bool seedToPoints);
+    bool chosenFromPoints() const { return chosenFromPoints_; }
 
     // -- drivers -------------------------------------------------------------------
     // Brute force over every segment independently (each its own group): choose, then
@@ -200,7 +218,35 @@ private:
     // returns false when the stream cannot be captured
     bool captureIterations(int batch, bool finalize, int capIter, hipGraphExec_t* exec);
     void ensureEvolveScratch();
-    void enqueueIteration(bool record, int slot = -1);
+    // per-iteration log targets of a recorded forward (labels of every position; counts and
+    // moved flags of every slot)
+    struct IterLog {
+        int* labels;
+        int* counts;
+        unsigned short* moved;
+    };
+    void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr);
+    // state of the last evolveForward
+    struct Tape {
+        long long iters = 0;
+        int every = 0;
+        int runs = 0;
+        int loggedRun = -1;      // run whose logs the buffers hold
+        int K = 0;
+        int64_t positions = 0;
+        T r = 0;
+        uint64_t version = 0;    // engine state the tape belongs to
+        DevBuf<T> ckpt;          // [runs][dim][sStride]
+        DevBuf<T> finalS;        // [dim][sStride]
+        DevBuf<int> labels;      // [every][positions]
+        DevBuf<int> counts;      // [every][K]
+        DevBuf<unsigned short> moved;
+        DevBuf<T> G, A;          // upstream gradient and step coefficients
+        bool valid = false;
+    } tape_;
+    uint64_t version_ = 0;       // bumped by every change of points, segments or chosen set
+    bool chosenFromPoints_ = false;
+    void recordRun(int run);
     void readEvents(int used);
     std::vector<hipEvent_t> events_;   // per batch slot: scan start, scan end, iteration end
     void fillSentinels();

@@ -42,6 +42,33 @@ def helix_curve(t: np.ndarray, pitch: float = 0.15) -> np.ndarray:
     return np.stack([np.cos(t), np.sin(t), pitch * t], axis=-1)
 
 
+def helix_loss(S: np.ndarray, turns: float = 5.0, pitch: float = 0.15):
+    """Sum of squared distances of the points S (K x 3) to the noise-free helix
+    t in [0, 2 pi turns], and its gradient with respect to S (both float64).
+
+    The nearest parameter of every point comes from a dense grid refined by Newton steps on
+    d/dt |s - h(t)|^2 / 2 = x sin t - y cos t + pitch (pitch t - z); at that parameter the
+    gradient of the squared distance is 2 (s - h(t*)) (the t-derivative vanishes there, or
+    t* sits on an end of the parameter range)."""
+    S = np.asarray(S, dtype=np.float64)
+    t_hi = 2.0 * math.pi * turns
+    grid = np.linspace(0.0, t_hi, int(t_hi / 0.05) + 1)
+    H = helix_curve(grid, pitch)
+    t = np.empty(len(S))
+    for a in range(0, len(S), 256):
+        blk = S[a:a + 256]
+        d2 = ((blk[:, None, :] - H[None, :, :]) ** 2).sum(-1)
+        t[a:a + 256] = grid[np.argmin(d2, axis=1)]
+    x, y, z = S[:, 0], S[:, 1], S[:, 2]
+    for _ in range(12):
+        g = x * np.sin(t) - y * np.cos(t) + pitch * (pitch * t - z)
+        h = x * np.cos(t) + y * np.sin(t) + pitch * pitch
+        step = np.where(h > 1e-12, g / np.where(h > 1e-12, h, 1.0), 0.0)
+        t = np.clip(t - step, 0.0, t_hi)
+    diff = S - helix_curve(t, pitch)
+    return float((diff * diff).sum()), 2.0 * diff
+
+
 def bezier_union(n: int, dim: int = 6, curves: int = 4, seed: int = 0, sigma: float = 0.02,
                  outlier_frac: float = 0.05, outlier_box: float = 1.2,
                  dtype=np.float32) -> np.ndarray:
