@@ -258,62 +258,91 @@ __global__ __launch_bounds__(256) void gather_members_kernel(const T* __restrict
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T lane_value(T v, int lane);
-template <>
-__device__ __forceinline__ float lane_value<float>(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+// Value of lane 16*(lane/16) + J, i.e. lane J of the caller's 16-lane row (DPP
+// row_newbcast, gfx90a and later).  Feeding it straight into an add makes the compiler
+// emit one v_add_f32_dpp.
+template <int J>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + J, 0xF, 0xF, false));
 }
-template <>
-__device__ __forceinline__ double lane_value<double>(double v, int lane) {
+template <int J>
+__device__ __forceinline__ double row_bcast(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x150 + J, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + J, 0xF, 0xF, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// sum[d] += member J of the current step, for J = 0..15 in order
+template <typename T, int DIM, int J>
+__device__ __forceinline__ void fold_row(T (&sum)[DIM], const T (&v)[DIM]) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + row_bcast<J>(v[d]);
+    if constexpr (J < 15) fold_row<T, DIM, J + 1>(sum, v);
+}
+
 // ---------------------------------------------------------------------------------
-// 4b. ordered sums + shift toward the mass centre: one wave per chosen point.  The wave
-// loads 64 consecutive members per step (coalesced) and folds them into the running sum
-// in member order, lane 0 first, exactly the reference's serial summation.
+// 4b. ordered sums + shift toward the mass centre: one 16-lane row per chosen point (four
+// per wave).  Every step the row loads 16 consecutive members (lane t: member c0 + t) and
+// folds them into the running sum in member order with row broadcasts, i.e. exactly the
+// reference's serial summation cordSums[k] += P[n] in sample order.  Lanes past the end
+// of a chain load +0: the running sum starts at +0 and a round-to-nearest sum is -0 only
+// when both addends are -0, so it is never -0 and adding +0 leaves it unchanged.
 // ---------------------------------------------------------------------------------
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
     if (*a.allDone) return;
-    const int64_t slot = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
-    if (slot >= a.totalSlots) return;
-    const int seg = a.slotSeg[slot];
-    if (seg < 0 || a.segDone[seg]) return;
-    const SegDesc sd = a.segs[seg];
-    const int k = (int)(slot - sd.sOff);
-    if (k >= a.segK[seg]) return;
-    const int lane = lane_id();
-    const int off = a.chainOff[slot];
-    const int cnt = a.chainEnd[slot] - off;
+    constexpr int kRow = 16;
+    const int t = threadIdx.x & (kRow - 1);
+    const int64_t slot = (int64_t)blockIdx.x * (256 / kRow) + threadIdx.x / kRow;
+    int seg = -1, off = 0, cnt = 0;
+    if (slot < a.totalSlots) {
+        seg = a.slotSeg[slot];
+        if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
+            off = a.chainOff[slot];
+            cnt = a.chainEnd[slot] - off;
+        } else {
+            seg = -1;
+        }
+    }
+    // the wave steps through the longest of its four chains; DPP needs every lane present
+    int steps = cnt;
+    steps = max(steps, __shfl_xor(steps, 16));
+    steps = max(steps, __shfl_xor(steps, 32));
     T sum[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) sum[d] = T(0);
+    // members stream through a ring of four 16-member steps, loaded three steps ahead;
+    // loads are unconditional (index clamped into the chain, or element 0 of the buffer
+    // for an empty one) so that no branch separates them from the folds
     const T* __restrict__ src = a.sortedP + off;
-    int c0 = 0;
-    for (; c0 + kWave <= cnt; c0 += kWave) {
-        T v[DIM];
+    const int last = max(cnt - 1, 0);
+    auto load = [&](T (&v)[DIM], int i) {
+        const int c = min(i, last);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) v[d] = src[d * a.mStride + c0 + lane];
-#pragma unroll
-        for (int q = 0; q < kWave; ++q)
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + lane_value<T>(v[d], q);
+        for (int d = 0; d < DIM; ++d) {
+            const T x = src[d * a.mStride + c];
+            v[d] = i < cnt ? x : T(0);
+        }
+    };
+    // uniform trip count (groups of four steps; steps past a chain's end fold +0)
+    const int groups = __builtin_amdgcn_readfirstlane((steps + 4 * kRow - 1) / (4 * kRow));
+    T b0[DIM], b1[DIM], b2[DIM], b3[DIM];
+    load(b0, t);
+    load(b1, kRow + t);
+    load(b2, 2 * kRow + t);
+    for (int g = 0; g < groups; ++g) {
+        const int c0 = g * 4 * kRow + t;
+        load(b3, c0 + 3 * kRow);
+        fold_row<T, DIM, 0>(sum, b0);
+        load(b0, c0 + 4 * kRow);
+        fold_row<T, DIM, 0>(sum, b1);
+        load(b1, c0 + 5 * kRow);
+        fold_row<T, DIM, 0>(sum, b2);
+        load(b2, c0 + 6 * kRow);
+        fold_row<T, DIM, 0>(sum, b3);
     }
-    if (c0 < cnt) {
-        const int m = cnt - c0;
-        T v[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) v[d] = lane < m ? src[d * a.mStride + c0 + lane] : T(0);
-        for (int q = 0; q < m; ++q)
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + lane_value<T>(v[d], q);
-    }
-    if (lane != 0) return;
+    if (seg < 0 || t != 0) return;
     if (a.outCounts) a.outCounts[slot] = cnt;
     if (a.outSums)
 #pragma unroll
@@ -420,7 +449,7 @@ hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int p
 
 template <typename T>
 hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, hipStream_t st) {
-    const int64_t blocks = (a.totalSlots + (256 / kWave) - 1) / (256 / kWave);
+    const int64_t blocks = (a.totalSlots + 15) / 16;
     if (blocks <= 0) return hipSuccess;
     const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;
