@@ -297,7 +297,7 @@ void Engine<T>::ensureEvolveScratch() {
     chainOff_.reserve(slots);
     chainEnd_.reserve(slots);
     counts_.reserve(slots);
-    moved_.reserve(slots);
+    moved_.reserve((size_t)slots * dim_);
     sums_.reserve((size_t)sStride_ * dim_);
     sortWs_.reserve(sort_workspace_bytes(totalP, (int)std::max<int64_t>(totalSlots_, 2)));
     sortedP_.reserve((size_t)totalP * dim_);
@@ -463,9 +463,12 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
     const int64_t totalP = compact() ? std::max<int64_t>(activeWork_, 1) : totalP_;
     // labels are live slots < keyBound_ (counts only shrink between refreshes)
     const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    check(stable_sort_by_key(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr,
-                             dN_.get(), totalP, (int)std::max<int64_t>(kb, 2),
-                             keys_.get(), members_.get(), dM_.get(), sortWs_.get(), st_, flags_.get()),
+    // members sorted by (label, position); the last pass writes their coordinates in
+    // chain order straight into sortedP_
+    const SortGather<T> sg{P_.get(), pStride_, pIndexPtr(), sortedP_.get(), totalP_, dim_};
+    check(stable_sort_by_key<T>(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr, dN_.get(),
+                                totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
+                                sortWs_.get(), st_, flags_.get(), sg),
           "sort");
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), kb, 0, flags_.get());
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainEnd_.get(), kb, 0, flags_.get());
@@ -475,8 +478,9 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
-                    log ? log->moved : (record ? moved_.get() : nullptr)};
-    check(launch_chain_sum_shift<T>(ha, dim_, totalP, st_), "chain_sum_shift");
+                    log ? log->moved : (record ? moved_.get() : nullptr),
+                    log ? log->movedStride : totalSlots_};
+    check(launch_chain_sum_shift<T>(ha, dim_, totalP, false, st_), "chain_sum_shift");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
 }
 
@@ -660,13 +664,18 @@ int Engine<T>::evolveIteration(std::vector<int>* labels, std::vector<int>* count
         check(hipMemcpyAsync(sums->data(), sums_.get(), sizeof(T) * sStride_ * dim_, hipMemcpyDeviceToHost, st_),
               "sums");
     }
+    std::vector<unsigned char> mv;
     if (moved) {
-        moved->resize((size_t)totalSlots_);
-        check(hipMemcpyAsync(moved->data(), moved_.get(), sizeof(unsigned short) * totalSlots_, hipMemcpyDeviceToHost,
-                             st_),
-              "moved");
+        mv.resize((size_t)totalSlots_ * dim_);
+        check(hipMemcpyAsync(mv.data(), moved_.get(), (size_t)totalSlots_ * dim_, hipMemcpyDeviceToHost, st_), "moved");
     }
     check(hipStreamSynchronize(st_), "evolveIteration");
+    if (moved) {
+        moved->assign((size_t)totalSlots_, 0);
+        for (int d = 0; d < dim_; ++d)
+            for (int64_t k = 0; k < totalSlots_; ++k)
+                if (mv[(size_t)d * totalSlots_ + k]) (*moved)[k] |= (unsigned short)(1u << d);
+    }
     int any = 0;
     for (int v : moved0) any |= v;
     return any;
@@ -682,7 +691,7 @@ void Engine<T>::recordRun(int run) {
     const int64_t kc = std::max(tape_.K, 1);
     for (int u = 0; u < steps; ++u) {
         IterLog lg{tape_.labels.get() + (int64_t)u * tape_.positions, tape_.counts.get() + (int64_t)u * kc,
-                   tape_.moved.get() + (int64_t)u * kc};
+                   tape_.moved.get() + (int64_t)u * dim_ * kc, kc};
         enqueueIteration(false, -1, &lg);
     }
 }
@@ -708,7 +717,7 @@ void Engine<T>::evolveForward(long long iters, int every) {
     tape_.finalS.reserve((size_t)sStride_ * dim_);
     tape_.labels.reserve((size_t)every * tape_.positions);
     tape_.counts.reserve((size_t)every * kc);
-    tape_.moved.reserve((size_t)every * kc);
+    tape_.moved.reserve((size_t)every * dim_ * kc);
     int zero = 0, f[2] = {0, 0};
     check(hipMemcpyAsync(dSegDone_.get(), &zero, sizeof(int), hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");

@@ -190,7 +190,7 @@ private:
     DevBuf<T> sums_, sortedP_;
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();
-    DevBuf<unsigned short> moved_;
+    DevBuf<unsigned char> moved_;   // [dim][totalSlots]
     DevBuf<unsigned char> sortWs_;
     DevBuf<int64_t> dN_, dM_;
     DevBuf<int> flags_;   // [0] allDone, [1] totalIters, [2] overflow
@@ -219,11 +219,12 @@ private:
     bool captureIterations(int batch, bool finalize, int capIter, hipGraphExec_t* exec);
     void ensureEvolveScratch();
     // per-iteration log targets of a recorded forward (labels of every position; counts and
-    // moved flags of every slot)
+    // per-coordinate moved flags of every slot, the flags at moved[d * movedStride + slot])
     struct IterLog {
         int* labels;
         int* counts;
-        unsigned short* moved;
+        unsigned char* moved;
+        int64_t movedStride;
     };
     void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr);
     // state of the last evolveForward
@@ -240,7 +241,7 @@ private:
         DevBuf<T> finalS;        // [dim][sStride]
         DevBuf<int> labels;      // [every][positions]
         DevBuf<int> counts;      // [every][K]
-        DevBuf<unsigned short> moved;
+        DevBuf<unsigned char> moved;   // [every][dim][K]
         DevBuf<T> G, A;          // upstream gradient and step coefficients
         bool valid = false;
     } tape_;

@@ -273,30 +273,34 @@ __device__ __forceinline__ double row_bcast(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// sum[d] += member J of the current step, for J = 0..15 in order
-template <typename T, int DIM, int J>
-__device__ __forceinline__ void fold_row(T (&sum)[DIM], const T (&v)[DIM]) {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) sum[d] = sum[d] + row_bcast<J>(v[d]);
-    if constexpr (J < 15) fold_row<T, DIM, J + 1>(sum, v);
+// sum += member J of the current step, for J = 0..15 in order
+template <typename T, int J>
+__device__ __forceinline__ void fold_row(T& sum, T v) {
+    sum = sum + row_bcast<J>(v);
+    if constexpr (J < 15) fold_row<T, J + 1>(sum, v);
 }
 
 // ---------------------------------------------------------------------------------
-// 4b. ordered sums + shift toward the mass centre: one 16-lane row per chosen point (four
-// per wave).  Every step the row loads 16 consecutive members (lane t: member c0 + t) and
-// folds them into the running sum in member order with row broadcasts, i.e. exactly the
-// reference's serial summation cordSums[k] += P[n] in sample order.  Lanes past the end
-// of a chain load +0: the running sum starts at +0 and a round-to-nearest sum is -0 only
-// when both addends are -0, so it is never -0 and adding +0 leaves it unchanged.
+// 4b. ordered sums + shift toward the mass centre: one 16-lane row per (coordinate,
+// chosen point), coordinate-major, so the coordinates of one point are summed by
+// different waves in parallel and a wave serves four points.  Every step the row loads
+// 16 consecutive members (lane t: member c0 + t) and folds them into the running sum in
+// member order with row broadcasts, i.e. exactly the reference's serial summation
+// cordSums[k][d] += P[n][d] in sample order.  Lanes past the end of a chain load +0: the
+// running sum starts at +0 and a round-to-nearest sum is -0 only when both addends are
+// -0, so it is never -0 and adding +0 leaves it unchanged.  The shift of one coordinate
+// depends only on that coordinate's sum (rd/cpu/brute_force/evolve.hpp:129-179).
 // ---------------------------------------------------------------------------------
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
     if (*a.allDone) return;
     constexpr int kRow = 16;
     const int t = threadIdx.x & (kRow - 1);
-    const int64_t slot = (int64_t)blockIdx.x * (256 / kRow) + threadIdx.x / kRow;
+    const int64_t row = (int64_t)blockIdx.x * (256 / kRow) + threadIdx.x / kRow;
+    const int d = (int)(row / max<int64_t>(a.totalSlots, 1));
+    const int64_t slot = row - (int64_t)d * a.totalSlots;
     int seg = -1, off = 0, cnt = 0;
-    if (slot < a.totalSlots) {
+    if (d < DIM) {
         seg = a.slotSeg[slot];
         if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
             off = a.chainOff[slot];
@@ -309,64 +313,53 @@ __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
     int steps = cnt;
     steps = max(steps, __shfl_xor(steps, 16));
     steps = max(steps, __shfl_xor(steps, 32));
-    T sum[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) sum[d] = T(0);
-    // members stream through a ring of four 16-member steps, loaded three steps ahead;
-    // loads are unconditional (index clamped into the chain, or element 0 of the buffer
-    // for an empty one) so that no branch separates them from the folds
-    const T* __restrict__ src = a.sortedP + off;
+    T sum = T(0);
+    // Members stream through a ring of kNB buffers of kQ 16-member steps (lane t holds
+    // members c + 16 q + t); a buffer is refilled kNB - 1 buffers ahead of its fold, enough
+    // to cover the L2 latency with the dependent adds.  Loads are unconditional (index
+    // clamped into the chain, or element 0 of the buffer for an empty one) so that no
+    // branch separates them from the folds; steps past a chain's end fold +0.
+    constexpr int kQ = 1, kNB = 16, kBuf = kQ * kRow;
+    const T* __restrict__ src = a.sortedP + (int64_t)min(d, DIM - 1) * a.mStride + off;
     const int last = max(cnt - 1, 0);
-    auto load = [&](T (&v)[DIM], int i) {
-        const int c = min(i, last);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const T x = src[d * a.mStride + c];
-            v[d] = i < cnt ? x : T(0);
-        }
+    auto load = [&](int i) {
+        const T x = src[min(i, last)];
+        return i < cnt ? x : T(0);
     };
-    // uniform trip count (groups of four steps; steps past a chain's end fold +0)
-    const int groups = __builtin_amdgcn_readfirstlane((steps + 4 * kRow - 1) / (4 * kRow));
-    T b0[DIM], b1[DIM], b2[DIM], b3[DIM];
-    load(b0, t);
-    load(b1, kRow + t);
-    load(b2, 2 * kRow + t);
-    for (int g = 0; g < groups; ++g) {
-        const int c0 = g * 4 * kRow + t;
-        load(b3, c0 + 3 * kRow);
-        fold_row<T, DIM, 0>(sum, b0);
-        load(b0, c0 + 4 * kRow);
-        fold_row<T, DIM, 0>(sum, b1);
-        load(b1, c0 + 5 * kRow);
-        fold_row<T, DIM, 0>(sum, b2);
-        load(b2, c0 + 6 * kRow);
-        fold_row<T, DIM, 0>(sum, b3);
+    const int nBuf = __builtin_amdgcn_readfirstlane((steps + kBuf - 1) / kBuf);   // uniform
+    T buf[kNB][kQ];
+#pragma unroll
+    for (int b = 0; b < kNB; ++b)
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) buf[b][q] = load(b * kBuf + q * kRow + t);
+    for (int g = 0; g * kNB < nBuf; ++g) {
+#pragma unroll
+        for (int b = 0; b < kNB; ++b) {
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) fold_row<T, 0>(sum, buf[b][q]);
+            const int next = ((g + 1) * kNB + b) * kBuf + t;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) buf[b][q] = load(next + q * kRow);
+        }
     }
     if (seg < 0 || t != 0) return;
-    if (a.outCounts) a.outCounts[slot] = cnt;
-    if (a.outSums)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) a.outSums[d * a.sStride + slot] = sum[d];
+    if (a.outCounts && d == 0) a.outCounts[slot] = cnt;
+    if (a.outSums) a.outSums[d * a.sStride + slot] = sum;
     bool moved = false;
-    unsigned mvMask = 0;
     if (cnt != 0) {
         const T fc = (T)cnt;
         constexpr T eps = FpTraits<T>::eps;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            T mc = sum[d] / fc;
-            T thr = T(2) * fabs(mc);
-            thr = thr * fc;
-            thr = thr * eps;
-            T& s = a.S[d * a.sStride + slot];
-            if (fabs(mc - s) > thr) {
-                s = mc;
-                moved = true;
-                mvMask |= 1u << d;
-            }
+        T mc = sum / fc;
+        T thr = T(2) * fabs(mc);
+        thr = thr * fc;
+        thr = thr * eps;
+        T& s = a.S[d * a.sStride + slot];
+        if (fabs(mc - s) > thr) {
+            s = mc;
+            moved = true;
         }
     }
-    if (a.outMoved) a.outMoved[slot] = (unsigned short)mvMask;
+    if (a.outMoved) a.outMoved[d * a.movedStride + slot] = moved ? 1 : 0;
     if (moved) a.segMoved[seg] = 1;
 }
 
@@ -448,14 +441,15 @@ hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int p
 }
 
 template <typename T>
-hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, hipStream_t st) {
-    const int64_t blocks = (a.totalSlots + 15) / 16;
+hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, bool gather, hipStream_t st) {
+    const int64_t blocks = ((int64_t)dim * a.totalSlots + 15) / 16;
     if (blocks <= 0) return hipSuccess;
     const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
-        hipLaunchKernelGGL((gather_members_kernel<T, DIM>), dim3(gblocks), dim3(256), 0, st, a.P, a.pStride,
-                           a.pIndex, a.members, a.mPtr, a.sortedP, a.mStride, a.allDone);
+        if (gather)
+            hipLaunchKernelGGL((gather_members_kernel<T, DIM>), dim3(gblocks), dim3(256), 0, st, a.P, a.pStride,
+                               a.pIndex, a.members, a.mPtr, a.sortedP, a.mStride, a.allDone);
         hipLaunchKernelGGL((chain_sum_shift_kernel<T, DIM>), dim3((unsigned)blocks), dim3(256), 0, st, a);
         ok = true;
     });
@@ -489,8 +483,8 @@ hipError_t launch_fill_sentinel(T* S, int64_t sStride, int dim, const SegDesc* s
 
 template hipError_t launch_evolve_scan<float>(const ScanArgs<float>&, int, int, int, int, hipStream_t);
 template hipError_t launch_evolve_scan<double>(const ScanArgs<double>&, int, int, int, int, hipStream_t);
-template hipError_t launch_chain_sum_shift<float>(const ShiftArgs<float>&, int, int64_t, hipStream_t);
-template hipError_t launch_chain_sum_shift<double>(const ShiftArgs<double>&, int, int64_t, hipStream_t);
+template hipError_t launch_chain_sum_shift<float>(const ShiftArgs<float>&, int, int64_t, bool, hipStream_t);
+template hipError_t launch_chain_sum_shift<double>(const ShiftArgs<double>&, int, int64_t, bool, hipStream_t);
 template hipError_t launch_fill_sentinel<float>(float*, int64_t, int, const SegDesc*, const int*, int, hipStream_t);
 template hipError_t launch_fill_sentinel<double>(double*, int64_t, int, const SegDesc*, const int*, int, hipStream_t);
 

@@ -54,7 +54,8 @@ struct ShiftArgs {
     const int* allDone;
     int* outCounts;         // optional: members per slot
     T* outSums;             // optional: SoA coordinate sums per slot
-    unsigned short* outMoved;  // optional: per-slot bit mask of moved coordinates
+    unsigned char* outMoved;   // optional: 1 where coordinate d of a slot moved, at [d * movedStride + slot]
+    int64_t movedStride;
 };
 
 // persistentBlocks > 0: that many persistent blocks pull the work items from a device
@@ -62,10 +63,11 @@ struct ShiftArgs {
 template <typename T>
 hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, int persistentBlocks,
                               hipStream_t st);
-// Gathers the members' coordinates in chain order, then sums and shifts; slots
-// [0, a.totalSlots) are visited (pass the live bound, not the capacity).
+// Sums and shifts (after gathering the members' coordinates into chain order when
+// `gather`; otherwise a.sortedP already holds them); slots [0, a.totalSlots) are visited
+// (pass the live bound, not the capacity).
 template <typename T>
-hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, hipStream_t st);
+hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, bool gather, hipStream_t st);
 hipError_t launch_chain_bounds(const int* keys, const int64_t* dM, int64_t maxM, int* chainOff,
                                int* chainEnd, const int* allDone, hipStream_t st);
 hipError_t launch_evolve_finalize(int numSegs, int* segDone, int* segMoved, int* segIter, int maxIter,

@@ -129,10 +129,31 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(int* __restrict__ 
     }
 }
 
+// Where a ranked pair goes: its value into valsOut ...
+struct ValueSink {
+    int* vals;
+    __device__ __forceinline__ void put(int64_t dst, int v) const { vals[dst] = v; }
+};
+// ... or the coordinates of the point it names into a SoA output (last pass of the evolve sort)
+template <typename T, int DIM>
+struct GatherSink {
+    const T* P;
+    int64_t pStride;
+    const int* pIndex;
+    T* out;
+    int64_t outStride;
+    __device__ __forceinline__ void put(int64_t dst, int v) const {
+        const int64_t n = point_of(pIndex, v);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) out[d * outStride + dst] = P[d * pStride + n];
+    }
+};
+
+template <typename Sink>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const int* __restrict__ keysIn, const int* __restrict__ valsIn, const int64_t* __restrict__ nPtr,
     int shift, int bits, int numBlocks, const int* __restrict__ ghistScanned,
-    int* __restrict__ keysOut, int* __restrict__ valsOut, const int* __restrict__ skip) {
+    int* __restrict__ keysOut, Sink sink, const int* __restrict__ skip) {
     __shared__ int cnt[kWavesPerBlock][1 << kMaxDigitBits];
     if (skip && *skip) return;
     const int nb = 1 << bits;
@@ -191,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
             int dig = (k >> shift) & mask;
             int dst = cnt[wave][dig] + pos[c];
             keysOut[dst] = k;
-            valsOut[dst] = valsIn ? valsIn[i] : (int)i;
+            sink.put(dst, valsIn ? valsIn[i] : (int)i);
         }
     }
 }
@@ -226,9 +247,13 @@ size_t sort_workspace_bytes(int64_t maxN, int numKeys) {
     return sizeof(int) * (size_t)(L + scanBlocks + 2 * std::max<int64_t>(maxN, 1)) + 64;
 }
 
-hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
-                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
-                              hipStream_t stream, const int* skip) {
+namespace {
+
+// LSD passes; `lastScatter(inK, inV, n, shift, w, blocks, ghist)` launches the final pass.
+template <typename Last>
+hipError_t sort_passes(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN, int numKeys,
+                       int* keysOut, int* valsOut, int64_t* dM, void* workspace, hipStream_t stream,
+                       const int* skip, Last&& lastScatter) {
     if (maxN <= 0) return hipSuccess;
     int bits = 1;
     while ((int64_t(1) << bits) < numKeys) ++bits;
@@ -259,13 +284,55 @@ hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_
                            (int)scanBlocks, p == 0 ? dM : nullptr, skip);
         hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream,
                            ghist, L, partials, skip);
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
-                           inV, n, shift, w, (int)blocks, ghist, oK, oV, skip);
+        if (p == passes - 1)
+            lastScatter(inK, inV, n, shift, w, (int)blocks, ghist, oK, oV);
+        else
+            hipLaunchKernelGGL(radix_scatter_kernel<ValueSink>, dim3((unsigned)blocks), dim3(kThreads), 0, stream,
+                               inK, inV, n, shift, w, (int)blocks, ghist, oK, ValueSink{oV}, skip);
         inK = oK;
         inV = oV;
         n = dM;
     }
     return hipGetLastError();
 }
+
+}  // namespace
+
+hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
+                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
+                              hipStream_t stream, const int* skip) {
+    return sort_passes(keysIn, valsIn, dN, maxN, numKeys, keysOut, valsOut, dM, workspace, stream, skip,
+                       [&](const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks,
+                           const int* ghist, int* oK, int* oV) {
+                           hipLaunchKernelGGL(radix_scatter_kernel<ValueSink>, dim3((unsigned)blocks), dim3(kThreads),
+                                              0, stream, inK, inV, n, shift, w, blocks, ghist, oK, ValueSink{oV},
+                                              skip);
+                       });
+}
+
+template <typename T>
+hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
+                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
+                              hipStream_t stream, const int* skip, const SortGather<T>& g) {
+    bool ok = false;
+    hipError_t e = sort_passes(
+        keysIn, valsIn, dN, maxN, numKeys, keysOut, valsOut, dM, workspace, stream, skip,
+        [&](const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks, const int* ghist,
+            int* oK, int*) {
+            RD_DISPATCH_DIM(g.dim, {
+                hipLaunchKernelGGL((radix_scatter_kernel<GatherSink<T, DIM>>), dim3((unsigned)blocks), dim3(kThreads),
+                                   0, stream, inK, inV, n, shift, w, blocks, ghist, oK,
+                                   (GatherSink<T, DIM>{g.P, g.pStride, g.pIndex, g.out, g.outStride}), skip);
+                ok = true;
+            });
+        });
+    if (e != hipSuccess) return e;
+    return ok || maxN <= 0 ? hipSuccess : hipErrorInvalidValue;
+}
+
+template hipError_t stable_sort_by_key<float>(const int*, const int*, const int64_t*, int64_t, int, int*, int*,
+                                              int64_t*, void*, hipStream_t, const int*, const SortGather<float>&);
+template hipError_t stable_sort_by_key<double>(const int*, const int*, const int64_t*, int64_t, int, int*, int*,
+                                               int64_t*, void*, hipStream_t, const int*, const SortGather<double>&);
 
 }  // namespace rdhip

@@ -17,6 +17,23 @@ hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_
                               int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
                               hipStream_t stream, const int* skip = nullptr);
 
+// Optional companion output of the last pass: for every kept pair, in sorted order, the
+// coordinates of point point_of(pIndex, value) of the SoA cloud P (dim x pStride) go to
+// out[d * outStride + rank] instead of writing the value itself (valsOut is not written).
+template <typename T>
+struct SortGather {
+    const T* P;
+    int64_t pStride;
+    const int* pIndex;
+    T* out;
+    int64_t outStride;
+    int dim;
+};
+template <typename T>
+hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_t* dN, int64_t maxN,
+                              int numKeys, int* keysOut, int* valsOut, int64_t* dM, void* workspace,
+                              hipStream_t stream, const int* skip, const SortGather<T>& gather);
+
 // Exclusive prefix sum of a[0..L) in place; `partials` needs scan_partials_count(L) ints;
 // the grand total is written to *total when total is not null.
 int64_t scan_partials_count(int64_t L);

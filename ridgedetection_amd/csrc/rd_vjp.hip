@@ -28,11 +28,10 @@ __global__ __launch_bounds__(256) void vjp_coef_kernel(VjpArgs<T> a) {
     for (int d = 0; d < DIM; ++d) g[d] = a.G[d * a.gStride + k];
     for (int u = a.steps - 1; u >= 0; --u) {
         const int c = a.counts[u * a.cStride + k];
-        const unsigned m = a.moved[u * a.cStride + k];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             T coef = T(0);
-            if (c > 0 && ((m >> d) & 1u)) {
+            if (c > 0 && a.moved[((int64_t)u * DIM + d) * a.cStride + k]) {
                 coef = g[d] / (T)c;
                 g[d] = T(0);
             }

@@ -9,13 +9,13 @@ namespace rdhip {
 // One logged run of `steps` iterations, newest last.  Per step u:
 //   labels[u * lStride + i]  chosen slot of position i, or -1
 //   counts[u * cStride + k]  members of slot k
-//   moved[u * cStride + k]   bit d set when coordinate d of slot k moved
+//   moved[(u * dim + d) * cStride + k]   1 when coordinate d of slot k moved
 template <typename T>
 struct VjpArgs {
     const int* labels;
     int64_t lStride;
     const int* counts;
-    const unsigned short* moved;
+    const unsigned char* moved;
     int64_t cStride;
     int steps;
     int K;              // live slots (0 .. K-1)
