@@ -156,6 +156,81 @@ def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
             "fwd_plus_bwd_over_fwd": (f + b) / f, "fwd_plus_bwd_over_plain_evolve": (f + b) / p}
 
 
+TILED_NAMES = {0: "local tree, mixed RD", 1: "grouped tree, mixed RD", 2: "local tree, local RD",
+               4: "grouped tree, local RD", 5: "extended tiles, mixed RD", 6: "extended tiles, local RD"}
+
+
+def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0)):
+    """Tiled ridge detection on the helix (1M 3-D points, sigma 0.05, r1 0.2, r2 0.4, 8 tiles
+    per dimension, tiles split at 10 000 samples) and on the 6-D Bezier union (2M points, 4
+    curves, noise 0.02, 5% outliers, r1 0.08, r2 0.16, 4 tiles per dimension).  For each
+    algorithm: wall time, the phases, and the evolve iterations' algorithmic bytes (every
+    tile's own + neighbour samples read once per iteration, chosen points read, summed and
+    updated) over their time, against the 8 TB/s HBM peak."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context
+    out = {}
+    for w in workloads:
+        if w == "helix":
+            P = synth.helix3d(1_000_000, seed=2024, sigma=0.05, dtype=np.float32)
+            r1, r2, tpd, cap = 0.2, 0.4, [8, 8, 8], 10000
+        else:
+            P = synth.bezier_union(2_000_000, dim=6, seed=6, dtype=np.float32)
+            r1, r2, tpd, cap = 0.08, 0.16, [4] * 6, 10000
+        for alg in algorithms:
+            ctx = Context(P.shape[1], np.float32)
+            ctx.set_points(P)
+            log(f"bench: tiled {w} algorithm {alg}")
+            t0 = time.perf_counter()
+            res = ctx.tiled(alg, r1, r2, cap, tpd)
+            wall = time.perf_counter() - t0
+            t = ctx.timers()
+            ctx.close()
+            gbs = t["evolve_bytes"] / (t["evolve_ms"] * 1e-3) / 1e9 if t["evolve_ms"] > 0 else 0.0
+            out[f"{w}/{TILED_NAMES[alg]}"] = {
+                "points": len(P), "dim": P.shape[1], "ridge_points": len(res), "wall_s": wall,
+                "build_ms": t["build_ms"], "choose_ms": t["choose_ms"], "evolve_ms": t["evolve_ms"],
+                "decimate_ms": t["decimate_ms"], "evolve_iterations": t["evolve_iterations"],
+                "rounds": t["rounds"], "evolve_GBps": gbs, "evolve_frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    return out
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (the HIP runtime's copy kernel), repeated 3
+    times: the known-size reference the HBM byte counters of the profiled kernels are scaled
+    by (FETCH_SIZE should read `mib` MiB per copy, WRITE_SIZE the same)."""
+    import ctypes as C
+    from ridgedetection_amd._lib import load
+    load()                                   # the library's HIP runtime
+    hip = C.CDLL("libamdhip64.so.7")
+    n = int(mib) << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(n), 3) != 0:    # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
+def measured_traffic(n, dtype):
+    """HBM bytes per scan launch from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "evolve_scan_helix1M_f32_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("points") != n or t.get("dtype") != dtype:
+        return None
+    return t.get("traffic_bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,7 +242,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
+    ap.add_argument("--no-tiled", action="store_true", help="skip the tiled measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -231,7 +312,7 @@ def main():
         "config": {"workload": "helix 1M 3-D points, sigma 0.05, r 0.2, brute-force evolve iteration",
                    "points": n, "dim": d, "r": r, "chosen": K, "samples_per_lane": args.ppt},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic(n, args.dtype),
                      "kernel": "evolve_scan_kernel", "avg_launch_ms": scan_ms,
                      "flops_per_launch": flops},
         "secondary": secondary,
@@ -240,6 +321,8 @@ def main():
     if not args.no_vjp and dt == np.float32:
         log("bench: differentiated evolve (200k helix, 20 iterations)")
         result["secondary"]["vjp_helix_200k"] = vjp_helix()
+    if not args.no_tiled and dt == np.float32:
+        result["secondary"]["tiled"] = tiled_runs()
     if not args.no_batch:
         result["secondary"]["batched_512_clouds"] = batched_vs_looped()
     if not args.no_cpu:

@@ -86,6 +86,8 @@ typedef struct rdhip_timers {
     double iteration_ms;     /* HIP-event time of whole evolve iterations (RDHIP_OPT_PROFILE) */
     double forward_ms;       /* rdhip_evolve_fwd (wall time, stream synchronised) */
     double backward_ms;      /* rdhip_evolve_bwd without the staging of host buffers */
+    double evolve_bytes;     /* algorithmic bytes of the evolve iterations: per segment and
+                                iteration n*dim*sizeof(T) + K*(3*dim*sizeof(T) + 4) */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */

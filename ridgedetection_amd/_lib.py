@@ -41,7 +41,7 @@ class Timers(C.Structure):
                 ("evolve_ms", C.c_double), ("decimate_ms", C.c_double), ("refine_ms", C.c_double),
                 ("evolve_iterations", C.c_int64), ("rounds", C.c_int64),
                 ("scan_kernel_ms", C.c_double), ("scan_launches", C.c_int64), ("iteration_ms", C.c_double),
-                ("forward_ms", C.c_double), ("backward_ms", C.c_double)]
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("evolve_bytes", C.c_double)]
 
 
 _lib = None

@@ -268,6 +268,7 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->iteration_ms = e.timers.iterationMs;
         out->forward_ms = e.timers.forwardMs;
         out->backward_ms = e.timers.backwardMs;
+        out->evolve_bytes = e.timers.evolveBytes;
         return RDHIP_OK;
     });
 }

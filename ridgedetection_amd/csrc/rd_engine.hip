@@ -497,6 +497,16 @@ void Engine<T>::readEvents(int used) {
 }
 
 template <typename T>
+void Engine<T>::addEvolveBytes(const std::vector<int>& itersPerSeg) {
+    std::vector<int> K = countsHost();
+    double b = 0;
+    for (size_t s = 0; s < segs_.size(); ++s)
+        b += (double)itersPerSeg[s] *
+             ((double)segs_[s].nP * dim_ * sizeof(T) + (double)K[s] * (3.0 * dim_ * sizeof(T) + 4.0));
+    timers.evolveBytes += b;
+}
+
+template <typename T>
 void Engine<T>::evolveFixed(long long iters) {
     const int ns = (int)segs_.size();
     if (ns == 0 || iters <= 0) return;
@@ -523,6 +533,7 @@ void Engine<T>::evolveFixed(long long iters) {
         done += nb;
     }
     if (exec) (void)hipGraphExecDestroy(exec);
+    addEvolveBytes(std::vector<int>(ns, (int)std::min<long long>(iters, 0x7fffffff)));
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -627,6 +638,11 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
+    {
+        std::vector<int> segIter(ns);
+        check(hipMemcpy(segIter.data(), dSegIter_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "iterations");
+        addEvolveBytes(segIter);
+    }
     lastEvolveIters = iters;
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

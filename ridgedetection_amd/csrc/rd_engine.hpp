@@ -65,6 +65,10 @@ struct Timers {
     double iterationMs = 0;       // HIP-event time of whole evolve iterations (when profiling)
     double forwardMs = 0;         // evolveForward calls (stream synchronised at the end)
     double backwardMs = 0;        // evolveBackward calls, device buffers in and out
+    // algorithmic bytes of the evolve iterations run: per segment and iteration, its samples
+    // read once (nP * dim * sizeof(T)) and its chosen points read, summed and updated
+    // (K * (3 * dim * sizeof(T) + 4))
+    double evolveBytes = 0;
 };
 
 struct RunOptions {
@@ -251,6 +255,7 @@ private:
     void readEvents(int used);
     std::vector<hipEvent_t> events_;   // per batch slot: scan start, scan end, iteration end
     void fillSentinels();
+    void addEvolveBytes(const std::vector<int>& itersPerSeg);
 };
 
 }  // namespace rdhip
