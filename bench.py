@@ -238,7 +238,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--ppt", type=int, default=4)
+    ap.add_argument("--ppt", type=int, default=0, help="samples per lane (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
@@ -267,7 +267,8 @@ def main():
     S0, _ = ctx.choose(r)
     K = len(S0)
     log(f"bench: K={K} chosen points")
-    ctx.set_option(OPT_PPT, args.ppt)
+    if args.ppt:
+        ctx.set_option(OPT_PPT, args.ppt)
     ctx.set_option(OPT_PROFILE, 1)
     ctx.evolve_fixed(r, args.warmup)
     # every library call returns with its stream synchronised
@@ -310,7 +311,9 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic (seeded helix, resident in HBM)",
         "config": {"workload": "helix 1M 3-D points, sigma 0.05, r 0.2, brute-force evolve iteration",
-                   "points": n, "dim": d, "r": r, "chosen": K, "samples_per_lane": args.ppt},
+                   "points": n, "dim": d, "r": r, "chosen": K,
+                   "scan_shape": {"samples_per_lane": t_after["scan_ppt"], "threads_per_block": t_after["scan_threads"],
+                                  "autotuned": not args.ppt}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(n, args.dtype),
                      "kernel": "evolve_scan_kernel", "avg_launch_ms": scan_ms,

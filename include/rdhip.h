@@ -88,17 +88,35 @@ typedef struct rdhip_timers {
     double backward_ms;      /* rdhip_evolve_bwd without the staging of host buffers */
     double evolve_bytes;     /* algorithmic bytes of the evolve iterations: per segment and
                                 iteration n*dim*sizeof(T) + K*(3*dim*sizeof(T) + 4) */
+    int64_t scan_ppt;        /* evolve scan shape in use: samples per lane ... */
+    int64_t scan_threads;    /* ... and threads per block (after autotuning) */
+    double tune_ms;          /* time spent timing candidate scan shapes */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */
 enum rdhip_option {
-    RDHIP_OPT_PPT = 0,             /* samples per lane of the evolve scan: 1, 2 or 4 (default 4) */
+    RDHIP_OPT_PPT = 0,             /* samples per lane of the evolve scan: 1, 2 or 4 (setting it
+                                      turns the autotuner off) */
     RDHIP_OPT_ITERS_PER_SYNC = 1,  /* evolve iterations queued between convergence checks (8) */
     RDHIP_OPT_PROFILE = 2,         /* 1: time the scan kernel and iterations with HIP events */
     RDHIP_OPT_PERSISTENT = 3,      /* 1 (default): persistent scan blocks fed by a device work
                                       queue; 0: one block per work item */
-    RDHIP_OPT_GRAPH = 4            /* 1 (default): replay captured batches of evolve iterations
+    RDHIP_OPT_GRAPH = 4,           /* 1 (default): replay captured batches of evolve iterations
                                       (hipGraph) until the device convergence flag is set */
+    RDHIP_OPT_LOCAL = 5,           /* 1 (default): small segments (at most 1024 chosen points and
+                                      the next two limits) evolve inside one workgroup each, every
+                                      iteration in LDS -- all active segments when all fit the
+                                      sample limit, else those of at most 2048 samples;
+                                      0: batch-wide iterations only */
+    RDHIP_OPT_LOCAL_MAX_SAMPLES = 6,  /* sample limit of that path (default 16384, at most 65536) */
+    RDHIP_OPT_LOCAL_MAX_WORK = 7,     /* limit on samples x chosen points (default 2^22) */
+    RDHIP_OPT_SCAN_THREADS = 8,    /* threads per evolve-scan block: 128, 256 or 512 (setting it
+                                      turns the autotuner off) */
+    RDHIP_OPT_AUTOTUNE = 9         /* 1 (default): a single cloud's evolve, on first use for a
+                                      (value type, dimension, log2 samples, log2 chosen points)
+                                      class, times every (samples per lane x threads per block)
+                                      scan shape on the data and keeps the fastest (cached per
+                                      process); batches keep 4 x 256 */
 };
 
 int rdhip_version(void);

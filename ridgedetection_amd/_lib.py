@@ -33,7 +33,8 @@ EXPORTS = [
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
-OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH = 0, 1, 2, 3, 4
+OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
+OPT_LOCAL_MAX_SAMPLES, OPT_LOCAL_MAX_WORK, OPT_SCAN_THREADS, OPT_AUTOTUNE = 6, 7, 8, 9
 
 
 class Timers(C.Structure):
@@ -41,7 +42,8 @@ class Timers(C.Structure):
                 ("evolve_ms", C.c_double), ("decimate_ms", C.c_double), ("refine_ms", C.c_double),
                 ("evolve_iterations", C.c_int64), ("rounds", C.c_int64),
                 ("scan_kernel_ms", C.c_double), ("scan_launches", C.c_int64), ("iteration_ms", C.c_double),
-                ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("evolve_bytes", C.c_double)]
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("evolve_bytes", C.c_double),
+                ("scan_ppt", C.c_int64), ("scan_threads", C.c_int64), ("tune_ms", C.c_double)]
 
 
 _lib = None

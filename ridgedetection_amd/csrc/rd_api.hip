@@ -269,6 +269,9 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->forward_ms = e.timers.forwardMs;
         out->backward_ms = e.timers.backwardMs;
         out->evolve_bytes = e.timers.evolveBytes;
+        out->scan_ppt = e.timers.scanPpt;
+        out->scan_threads = e.timers.scanThreads;
+        out->tune_ms = e.timers.tuneMs;
         return RDHIP_OK;
     });
 }
@@ -346,10 +349,23 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
         switch (option) {
             case RDHIP_OPT_PPT:
                 if (value != 1 && value != 2 && value != 4) return fail(RDHIP_ERR_INVALID, "ppt must be 1, 2 or 4");
+                e.options.autotune = false;
                 if (e.options.ppt != (int)value) {
                     e.options.ppt = (int)value;
                     e.rebuildWorkItems();
                 }
+                return RDHIP_OK;
+            case RDHIP_OPT_SCAN_THREADS:
+                if (value != 128 && value != 256 && value != 512)
+                    return fail(RDHIP_ERR_INVALID, "scan threads must be 128, 256 or 512");
+                e.options.autotune = false;
+                if (e.options.scanThreads != (int)value) {
+                    e.options.scanThreads = (int)value;
+                    e.rebuildWorkItems();
+                }
+                return RDHIP_OK;
+            case RDHIP_OPT_AUTOTUNE:
+                e.options.autotune = value != 0;
                 return RDHIP_OK;
             case RDHIP_OPT_ITERS_PER_SYNC:
                 if (value < 1 || value > 1024) return fail(RDHIP_ERR_INVALID, "iterations per sync must be in [1, 1024]");
@@ -363,6 +379,17 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
                 return RDHIP_OK;
             case RDHIP_OPT_GRAPH:
                 e.options.graph = value != 0;
+                return RDHIP_OK;
+            case RDHIP_OPT_LOCAL:
+                e.options.local = value != 0;
+                return RDHIP_OK;
+            case RDHIP_OPT_LOCAL_MAX_SAMPLES:
+                if (value < 0 || value > kLocalMaxP) return fail(RDHIP_ERR_INVALID, "local sample limit must be in [0, 65536]");
+                e.options.localMaxP = (int)value;
+                return RDHIP_OK;
+            case RDHIP_OPT_LOCAL_MAX_WORK:
+                if (value < 0) return fail(RDHIP_ERR_INVALID, "negative local work limit");
+                e.options.localMaxWork = value;
                 return RDHIP_OK;
         }
         return fail(RDHIP_ERR_INVALID, "unknown option");

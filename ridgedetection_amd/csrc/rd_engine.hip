@@ -10,7 +10,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 
 namespace rdhip {
 
@@ -219,7 +222,7 @@ template <typename T>
 void Engine<T>::buildWorkItems() {
     std::vector<WorkItem>& items = items_;
     items.clear();
-    const int per = kScanThreads * std::max(1, options.ppt);
+    const int per = std::max(64, options.scanThreads) * std::max(1, options.ppt);
     for (int s = 0; s < (int)segs_.size(); ++s)
         for (int st = 0; st < segs_[s].nP; st += per) items.push_back({s, st, std::min(segs_[s].nP, st + per), 0});
     numItems_ = (int)items.size();
@@ -455,7 +458,8 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
     }
     // one segment: a block per work item; batches of segments: persistent blocks on the
     // active-item queue
-    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, compact() ? persistentBlocks() : 0, st_),
+    check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads,
+                                compact() ? persistentBlocks() : 0, st_),
           "evolve_scan");
     if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
     // positions the sort and the member passes cover: the dense active range of a batch
@@ -496,6 +500,88 @@ void Engine<T>::readEvents(int used) {
     }
 }
 
+namespace {
+std::mutex g_tuneMutex;
+std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> g_tuned;   // -> (ppt, threads)
+inline int ilog2(int64_t v) {
+    int l = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++l;
+    }
+    return l;
+}
+}  // namespace
+
+template <typename T>
+void Engine<T>::autotuneScan() {
+    // Batches of segments keep the default shape: their iterations are dominated by the
+    // per-iteration passes, and the item size also sets the granularity of the active list.
+    if (!options.autotune || segs_.size() != 1) {
+        timers.scanPpt = options.ppt;
+        timers.scanThreads = options.scanThreads;
+        return;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const std::vector<int> K = countsHost();
+    int64_t kTotal = 0;
+    for (int k : K) kTotal += k;
+    const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)),
+                                     ilog2(kTotal + 1), 0);
+    std::pair<int, int> best{options.ppt, options.scanThreads};
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> g(g_tuneMutex);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        static const int cands[][2] = {{4, 256}, {2, 256}, {1, 256}, {4, 128}, {2, 128}, {1, 128},
+                                       {4, 512}, {2, 512}, {1, 512}};
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+                       dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
+                       nullptr, nullptr, nullptr};
+        float bestMs = 1e30f;
+        for (const auto& c : cands) {
+            options.ppt = c[0];
+            options.scanThreads = c[1];
+            buildWorkItems();
+            sa.items = dItems_.get();
+            check(launch_evolve_scan<T>(sa, dim_, numItems_, c[0], c[1], 0, st_), "tune");
+            check(hipEventRecord(e0, st_), "event");
+            for (int r = 0; r < 3; ++r) check(launch_evolve_scan<T>(sa, dim_, numItems_, c[0], c[1], 0, st_), "tune");
+            check(hipEventRecord(e1, st_), "event");
+            check(hipEventSynchronize(e1), "event");
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, e0, e1), "event");
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = {c[0], c[1]};
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> g(g_tuneMutex);
+        g_tuned[key] = best;
+    }
+    if (options.ppt != best.first || options.scanThreads != best.second || !cached) {
+        options.ppt = best.first;
+        options.scanThreads = best.second;
+        buildWorkItems();
+        if (segs_.size() > 1) orderItemsByCost();
+        check(hipStreamSynchronize(st_), "autotune");
+    }
+    timers.scanPpt = options.ppt;
+    timers.scanThreads = options.scanThreads;
+    timers.tuneMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 template <typename T>
 void Engine<T>::addEvolveBytes(const std::vector<int>& itersPerSeg) {
     std::vector<int> K = countsHost();
@@ -516,6 +602,7 @@ void Engine<T>::evolveFixed(long long iters) {
     int f[2] = {0, 0};
     check(hipMemcpyAsync(dSegDone_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    autotuneScan();
     if (compact()) rebuildActive(zeros);
     const int batch = std::max(1, options.itersPerSync);
     long long done = 0;
@@ -576,6 +663,59 @@ bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGrap
 }
 
 template <typename T>
+int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
+    const int ns = (int)segs_.size();
+    std::vector<int> K = countsHost();
+    // Taking every active segment off the batch-wide iterations removes their fixed
+    // per-iteration cost entirely; taking only some keeps it, so then only the smallest go.
+    auto eligible = [&](int s, int maxP) {
+        return segs_[s].nP <= std::min(kLocalMaxP, maxP) && K[s] <= kLocalMaxK &&
+               (int64_t)segs_[s].nP * K[s] <= options.localMaxWork;
+    };
+    bool all = true;
+    for (int s = 0; s < ns; ++s) all = all && (done[s] || eligible(s, options.localMaxP));
+    const int limitP = all ? options.localMaxP : std::min(options.localMaxP, options.localSmallP);
+    std::vector<int> cand;
+    for (int s = 0; s < ns; ++s)
+        if (!done[s] && eligible(s, limitP)) cand.push_back(s);
+    if (cand.empty()) return 0;
+    // the group shares one LDS layout sized by its largest K (and the chunk, bounded by the
+    // largest sample count): take segments by increasing K while the layout fits
+    std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return K[x] < K[y]; });
+    std::vector<int> list;
+    int maxK = 0, maxP = 0;
+    for (int s : cand) {
+        const int k2 = std::max(maxK, K[s]), p2 = std::max(maxP, (int)segs_[s].nP);
+        const int ks = (int)round_up(std::max(k2, 1), kChosenTile);
+        if (local_lds_bytes(dim_, sizeof(T), ks, std::min(std::max(p2, 1), kLocalChunk)) > kLocalLdsBudget) break;
+        maxK = k2;
+        maxP = p2;
+        list.push_back(s);
+    }
+    if (list.empty()) return 0;
+    // longest first: cost ~ samples x chosen points
+    std::stable_sort(list.begin(), list.end(), [&](int x, int y) {
+        return (int64_t)segs_[x].nP * std::max(K[x], 1) > (int64_t)segs_[y].nP * std::max(K[y], 1);
+    });
+    dLocalList_.reserve(list.size());
+    check(hipMemcpyAsync(dLocalList_.get(), list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, st_),
+          "local list");
+    int zero2[2] = {0, 0};
+    check(hipMemcpyAsync(flags_.get() + 5, zero2, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "local flags");
+    LocalArgs<T> a{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegR1Sqr_.get(),
+                   dLocalList_.get(), (int)list.size(), (int)round_up(std::max(maxK, 1), kChosenTile),
+                   std::min(std::max(maxP, 1), kLocalChunk), maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0,
+                   (int)kEvolveGuard, dSegIter_.get(), flags_.get() + 5, flags_.get() + 6};
+    check(launch_local_evolve<T>(a, dim_, st_), "local_evolve");
+    int res[2] = {0, 0};
+    check(hipMemcpyAsync(res, flags_.get() + 5, sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "local flags");
+    check(hipStreamSynchronize(st_), "local_evolve");
+    if (res[1]) throw std::runtime_error("evolve did not converge within the iteration guard");
+    for (int s : list) done[s] = 1;
+    return res[0];
+}
+
+template <typename T>
 long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) {
     const int ns = (int)segs_.size();
     if (ns == 0) return 0;
@@ -589,10 +729,24 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     }
     if (!anyActive) return 0;
     int f[2] = {0, 0};
-    check(hipMemcpyAsync(dSegDone_.get(), done.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(dSegIter_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "iter");
+    // small segments iterate inside one workgroup each; the rest share batch-wide iterations
+    const int localIters = options.local ? evolveLocal(maxIter, done) : 0;
+    bool anyGlobal = false;
+    for (int s = 0; s < ns; ++s) anyGlobal |= !done[s];
+    if (!anyGlobal) {
+        std::vector<int> segIter(ns);
+        check(hipMemcpy(segIter.data(), dSegIter_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "iterations");
+        addEvolveBytes(segIter);
+        lastEvolveIters = localIters;
+        timers.evolveIterations += localIters;
+        timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return localIters;
+    }
+    check(hipMemcpyAsync(dSegDone_.get(), done.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    autotuneScan();
     if (compact()) rebuildActive(done);
     const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
     long long iters = 0;
@@ -643,6 +797,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         check(hipMemcpy(segIter.data(), dSegIter_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "iterations");
         addEvolveBytes(segIter);
     }
+    iters = std::max<long long>(iters, localIters);
     lastEvolveIters = iters;
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -737,6 +892,7 @@ void Engine<T>::evolveForward(long long iters, int every) {
     int zero = 0, f[2] = {0, 0};
     check(hipMemcpyAsync(dSegDone_.get(), &zero, sizeof(int), hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    autotuneScan();
     for (int run = 0; run < tape_.runs; ++run) {
         check(hipMemcpyAsync(tape_.ckpt.get() + (size_t)run * sStride_ * dim_, S_.get(), sBytes,
                              hipMemcpyDeviceToDevice, st_),

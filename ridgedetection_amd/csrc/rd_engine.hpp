@@ -6,6 +6,7 @@
 #include "rd_common.hpp"
 #include "rd_decimate.hpp"
 #include "rd_evolve.hpp"
+#include "rd_local.hpp"
 #include "rd_sort.hpp"
 #include "rd_vjp.hpp"
 
@@ -69,16 +70,24 @@ struct Timers {
     // read once (nP * dim * sizeof(T)) and its chosen points read, summed and updated
     // (K * (3 * dim * sizeof(T) + 4))
     double evolveBytes = 0;
+    int scanPpt = 0, scanThreads = 0;   // scan shape in use (after autotuning)
+    double tuneMs = 0;                  // time spent timing candidate shapes
 };
 
 struct RunOptions {
     long long maxEvolveIters = 0;   // per evolve call; 0 = until convergence (reference)
     long long maxRounds = 0;        // 0 = until the count is stable (reference)
     int ppt = 4;                    // samples per lane in the evolve scan
+    int scanThreads = 256;          // threads per scan block (128, 256, 512)
+    bool autotune = true;           // pick ppt x scanThreads by timing the scan on first use
     int itersPerSync = 8;           // evolve iterations enqueued between convergence checks
     bool profile = false;           // record HIP events around the scan kernel / iterations
-    bool persistent = true;         // scan as persistent blocks fed by a device work queue
+    bool persistent = false;        // scan as persistent blocks fed by a device work queue
     bool graph = true;              // replay captured batches of iterations (hipGraph)
+    bool local = true;              // evolve small segments inside one workgroup each
+    int localMaxP = 16384;          // ... when every active segment has at most this many samples
+    int localSmallP = 2048;         // ... otherwise only those with at most this many
+    int64_t localMaxWork = 1 << 22; // ... and samples x chosen points at most this
 };
 
 template <typename T>
@@ -255,7 +264,15 @@ private:
     void readEvents(int used);
     std::vector<hipEvent_t> events_;   // per batch slot: scan start, scan end, iteration end
     void fillSentinels();
+    // Single-segment runs: times the scan kernel for every (samples per lane, block size)
+    // candidate on the current cloud and chosen set and keeps the fastest; choices are
+    // cached per (value type, dimension, log2 samples, log2 chosen points).
+    void autotuneScan();
     void addEvolveBytes(const std::vector<int>& itersPerSeg);
+    // Evolves the small active segments with the segment-local kernel and marks them done;
+    // returns the largest iteration count among them (0 when none).
+    int evolveLocal(long long maxIter, std::vector<int>& done);
+    DevBuf<int> dLocalList_;
 };
 
 }  // namespace rdhip

@@ -58,11 +58,12 @@ struct ShiftArgs {
     int64_t movedStride;
 };
 
-// persistentBlocks > 0: that many persistent blocks pull the work items from a device
-// queue; 0: one block per work item.
+// persistentBlocks > 0: that many persistent blocks (of kScanThreads) pull the work items
+// from a device queue; 0: one block of `threads` (128, 256 or 512) per work item, each item
+// holding threads x ppt samples.
 template <typename T>
-hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, int persistentBlocks,
-                              hipStream_t st);
+hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int ppt, int threads,
+                              int persistentBlocks, hipStream_t st);
 // Sums and shifts (after gathering the members' coordinates into chain order when
 // `gather`; otherwise a.sortedP already holds them); slots [0, a.totalSlots) are visited
 // (pass the live bound, not the capacity).

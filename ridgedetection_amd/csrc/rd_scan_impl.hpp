@@ -1,0 +1,128 @@
+// rd_scan_impl.hpp -- the evolve scan kernels (nearest chosen point of every sample) and
+// their launcher, instantiated per value type by rd_scan_f32.hip / rd_scan_f64.hip so the
+// (dimension x samples per lane x block size) variants compile in parallel.
+#pragma once
+
+#include "rd_evolve.hpp"
+#include "rd_nearest.hpp"
+
+namespace rdhip {
+
+// ---------------------------------------------------------------------------------
+// 1. nearest chosen point
+// ---------------------------------------------------------------------------------
+template <typename T, int DIM, int PPT, int TH>
+__device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem it) {
+    const SegDesc sd = a.segs[it.seg];
+    const int tid = threadIdx.x;
+    int* __restrict__ lab = a.labels + sd.pOff;
+    // compact output: slot of sample i is it.pad + (i - it.start)
+    int* __restrict__ ck = a.outKeys ? a.outKeys + it.pad - it.start : nullptr;
+    int* __restrict__ cp = a.outKeys ? a.outPos + it.pad - it.start : nullptr;
+    auto put = [&](int i, int label) {
+        if (ck) {
+            ck[i] = label;
+            cp[i] = (int)(sd.pOff + i);
+        } else {
+            lab[i] = label;
+        }
+    };
+    if (a.segDone[it.seg]) {
+        for (int i = it.start + tid; i < it.end; i += TH) put(i, -1);
+        return;
+    }
+    const int K = a.segK[it.seg];
+    T p[PPT][DIM];
+    bool valid[PPT];
+    int idx[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        int i = it.start + tid + j * TH;
+        idx[j] = i;
+        valid[j] = i < it.end;
+        const int64_t src = valid[j] ? point_of(a.pIndex, sd.pOff + i) : 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) p[j][d] = valid[j] ? a.P[d * a.pStride + src] : T(0);
+    }
+    int label[PPT];
+    nearest_labels<T, DIM, PPT>(p, valid, K, a.S + sd.sOff, a.sStride, a.segRSqr[it.seg], label);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j)
+        if (valid[j]) put(idx[j], label[j] < 0 ? -1 : (int)(sd.sOff + label[j]));
+}
+
+// one block per work item (blocks past a device-side item count exit)
+template <typename T, int DIM, int PPT, int TH>
+__global__ __launch_bounds__(TH) void evolve_scan_kernel(ScanArgs<T> a) {
+    if (*a.allDone) return;
+    if (a.numItemsDev && (int)blockIdx.x >= *a.numItemsDev) return;
+    scan_item<T, DIM, PPT, TH>(a, a.items[blockIdx.x]);
+}
+
+// persistent blocks pulling work items from a device queue (items are ordered by
+// decreasing cost, so the longest segments start first)
+template <typename T, int DIM, int PPT>
+__global__ __launch_bounds__(kScanThreads) void evolve_scan_persistent_kernel(ScanArgs<T> a, int numItems) {
+    __shared__ int next;
+    if (*a.allDone) return;
+    if (a.numItemsDev) numItems = *a.numItemsDev;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = next;
+        __syncthreads();
+        if (item >= numItems) break;
+        scan_item<T, DIM, PPT, kScanThreads>(a, a.items[item]);
+    }
+}
+
+// per-iteration reset of the work queue of the persistent form
+namespace {
+__global__ void iteration_prologue_kernel(int* __restrict__ queue, const int* __restrict__ allDone) {
+    if (*allDone) return;
+    *queue = 0;
+}
+}  // namespace
+
+template <typename T, int DIM, int PPT, int TH>
+void launch_scan_shape(const ScanArgs<T>& a, int numItems, int persistentBlocks, hipStream_t st) {
+    if (persistentBlocks > 0) {
+        hipLaunchKernelGGL(iteration_prologue_kernel, dim3(1), dim3(1), 0, st, a.queue, a.allDone);
+        hipLaunchKernelGGL((evolve_scan_persistent_kernel<T, DIM, PPT>), dim3(std::min(persistentBlocks, numItems)),
+                           dim3(kScanThreads), 0, st, a, numItems);
+    } else {
+        hipLaunchKernelGGL((evolve_scan_kernel<T, DIM, PPT, TH>), dim3(numItems), dim3(TH), 0, st, a);
+    }
+}
+
+template <typename T, int DIM, int PPT>
+bool launch_scan_threads(const ScanArgs<T>& a, int numItems, int threads, int persistentBlocks, hipStream_t st) {
+    if (persistentBlocks > 0 || threads == 256) {
+        launch_scan_shape<T, DIM, PPT, 256>(a, numItems, persistentBlocks, st);
+    } else if (threads == 128) {
+        launch_scan_shape<T, DIM, PPT, 128>(a, numItems, 0, st);
+    } else if (threads == 512) {
+        launch_scan_shape<T, DIM, PPT, 512>(a, numItems, 0, st);
+    } else {
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+hipError_t launch_evolve_scan_impl(const ScanArgs<T>& a, int dim, int numItems, int ppt, int threads,
+                                   int persistentBlocks, hipStream_t st) {
+    if (numItems <= 0) return hipSuccess;
+    bool ok = false;
+    RD_DISPATCH_DIM(dim, {
+        if (ppt == 4)
+            ok = launch_scan_threads<T, DIM, 4>(a, numItems, threads, persistentBlocks, st);
+        else if (ppt == 2)
+            ok = launch_scan_threads<T, DIM, 2>(a, numItems, threads, persistentBlocks, st);
+        else if (ppt == 1)
+            ok = launch_scan_threads<T, DIM, 1>(a, numItems, threads, persistentBlocks, st);
+    });
+    return ok ? hipGetLastError() : hipErrorInvalidValue;
+}
+
+}  // namespace rdhip

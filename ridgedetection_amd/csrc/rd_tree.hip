@@ -245,6 +245,64 @@ __global__ __launch_bounds__(256) void seg_bounds_kernel(const T* __restrict__ P
     }
 }
 
+// Tight bounds of the whole cloud P[:, 0..n): per-block partial min/max (order-free, so
+// exact), then one block combines the partials.
+template <typename T>
+__device__ __forceinline__ void block_minmax(T& mn, T& mx, T* smn, T* smx) {
+    smn[threadIdx.x] = mn;
+    smx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const T a = smn[threadIdx.x + h], b = smx[threadIdx.x + h];
+            smn[threadIdx.x] = (a < smn[threadIdx.x]) ? a : smn[threadIdx.x];
+            smx[threadIdx.x] = (b > smx[threadIdx.x]) ? b : smx[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    mn = smn[0];
+    mx = smx[0];
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cloud_bounds_partial_kernel(const T* __restrict__ P, int64_t ps, int dim,
+                                                                   int64_t n, T* __restrict__ partial) {
+    __shared__ T smn[256], smx[256];
+    for (int d = 0; d < dim; ++d) {
+        T mn = FpTraits<T>::big, mx = -FpTraits<T>::big;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+            const T v = P[d * ps + i];
+            mn = (v < mn) ? v : mn;
+            mx = (v > mx) ? v : mx;
+        }
+        block_minmax(mn, mx, smn, smx);
+        if (threadIdx.x == 0) {
+            partial[((int64_t)blockIdx.x * dim + d) * 2] = mn;
+            partial[((int64_t)blockIdx.x * dim + d) * 2 + 1] = mx;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cloud_bounds_final_kernel(const T* __restrict__ partial, int nb, int dim,
+                                                                 T* __restrict__ out) {
+    __shared__ T smn[256], smx[256];
+    for (int d = 0; d < dim; ++d) {
+        T mn = FpTraits<T>::big, mx = -FpTraits<T>::big;
+        for (int b = threadIdx.x; b < nb; b += 256) {
+            const T a = partial[((int64_t)b * dim + d) * 2], c = partial[((int64_t)b * dim + d) * 2 + 1];
+            mn = (a < mn) ? a : mn;
+            mx = (c > mx) ? c : mx;
+        }
+        block_minmax(mn, mx, smn, smx);
+        if (threadIdx.x == 0) {
+            out[2 * d] = mn;
+            out[2 * d + 1] = mx;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void copy_ranges_kernel(const RangeCopy* __restrict__ rc, int* __restrict__ dst) {
     const RangeCopy r = rc[blockIdx.x];
     for (int64_t i = threadIdx.x; i < r.cnt; i += 256) dst[r.dstOff + i] = r.src[r.srcOff + i];
@@ -321,18 +379,14 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
     hipLaunchKernelGGL((iota_kernel<T>), dim3(grid_of(n)), dim3(256), 0, st_, ident, n);
     HTile root;
     {
-        DevBuf<int> rOff, rEnd, rSel;
-        int z = 0, nn = (int)n;
-        rOff.reserve(1);
-        rEnd.reserve(1);
-        rSel.reserve(1);
-        check(hipMemcpyAsync(rOff.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
-        check(hipMemcpyAsync(rEnd.get(), &nn, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
-        check(hipMemcpyAsync(rSel.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "bounds");
-        DevBuf<T> out;
+        const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 4095) / 4096));
+        DevBuf<T> out, partial;
         out.reserve(2 * dim);
-        hipLaunchKernelGGL((seg_bounds_kernel<T>), dim3(1), dim3(256), 0, st_, P, ps, dim, ident, rOff.get(),
-                           rEnd.get(), rSel.get(), out.get());
+        partial.reserve((size_t)nb * 2 * dim);
+        hipLaunchKernelGGL((cloud_bounds_partial_kernel<T>), dim3(nb), dim3(256), 0, st_, P, ps, dim, n,
+                           partial.get());
+        hipLaunchKernelGGL((cloud_bounds_final_kernel<T>), dim3(1), dim3(256), 0, st_, partial.get(), nb, dim,
+                           out.get());
         std::vector<T> hb(2 * dim);
         check(hipMemcpyAsync(hb.data(), out.get(), sizeof(T) * 2 * dim, hipMemcpyDeviceToHost, st_), "bounds");
         check(hipStreamSynchronize(st_), "bounds");

@@ -110,3 +110,27 @@ def test_decimate_nocopy_rule(oracle):
         SD, keep = c.decimate(float(g["r2"]), rule=1)
     np.testing.assert_array_equal(keep, g["alive"])
     assert len(SD) == int(g["count"])
+
+
+def test_scan_shapes_and_autotuner(oracle):
+    """Every evolve-scan shape (samples per lane x threads per block) gives the oracle's
+    result; the autotuner picks one of them and reports it."""
+    from ridgedetection_amd._lib import Context, OPT_PPT, OPT_SCAN_THREADS
+    P = synth.helix3d(30000, seed=12)
+    ref, _ = oracle.ridge_detection(P, 0.2, 0.4)
+    for ppt in (1, 2, 4):
+        for threads in (128, 256, 512):
+            _say(f"case scan shape ppt={ppt} threads={threads}")
+            with Context(3, np.float32) as c:
+                c.set_option(OPT_PPT, ppt)
+                c.set_option(OPT_SCAN_THREADS, threads)
+                c.set_points(P)
+                np.testing.assert_array_equal(c.ridge_detection(0.2, 0.4), ref)
+                t = c.timers()
+                assert (t["scan_ppt"], t["scan_threads"]) == (ppt, threads)
+    _say("case autotuned scan shape")
+    with Context(3, np.float32) as c:
+        c.set_points(P)
+        np.testing.assert_array_equal(c.ridge_detection(0.2, 0.4), ref)
+        t = c.timers()
+        assert t["scan_ppt"] in (1, 2, 4) and t["scan_threads"] in (128, 256, 512)
