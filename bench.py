@@ -41,7 +41,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P, S, r, budget_s=20.0):
+def cpu_baseline(P, S, r, budget_s=24.0):
     """Times the reference's OpenMP evolve pass (calculateClosestSphereCenter_omp +
     shiftTowardMassCenter_omp) over the same cloud and chosen set."""
     from oracle.oracle import Oracle, available
@@ -61,7 +61,7 @@ def cpu_baseline(P, S, r, budget_s=20.0):
         o.evolve_omp_iter(P[:m], S, r)
         reps += 1
         el = time.perf_counter() - t0
-        if el > budget_s / 2 or reps >= 3:
+        if el > budget_s / 2 or reps >= 25:
             break
     value = reps * m * k / el
     return {"value": value, "unit": "pair-evals/s", "cores": threads, "kind": kind,

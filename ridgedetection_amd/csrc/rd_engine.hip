@@ -547,24 +547,34 @@ void Engine<T>::autotuneScan() {
         ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                        dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
                        nullptr, nullptr, nullptr};
-        float bestMs = 1e30f;
-        for (const auto& c : cands) {
-            options.ppt = c[0];
-            options.scanThreads = c[1];
-            buildWorkItems();
-            sa.items = dItems_.get();
-            check(launch_evolve_scan<T>(sa, dim_, numItems_, c[0], c[1], 0, st_), "tune");
-            check(hipEventRecord(e0, st_), "event");
-            for (int r = 0; r < 3; ++r) check(launch_evolve_scan<T>(sa, dim_, numItems_, c[0], c[1], 0, st_), "tune");
-            check(hipEventRecord(e1, st_), "event");
-            check(hipEventSynchronize(e1), "event");
-            float ms = 0;
-            check(hipEventElapsedTime(&ms, e0, e1), "event");
-            if (ms < bestMs) {
-                bestMs = ms;
-                best = {c[0], c[1]};
+        // three rounds over all candidates (so that no shape always runs first), each
+        // launch pair timed after one untimed launch; a shape scores its best round
+        constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
+        float score[kCands];
+        for (int c = 0; c < kCands; ++c) score[c] = 1e30f;
+        for (int round = 0; round < 3; ++round) {
+            for (int c = 0; c < kCands; ++c) {
+                options.ppt = cands[c][0];
+                options.scanThreads = cands[c][1];
+                buildWorkItems();
+                sa.items = dItems_.get();
+                check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
+                check(hipEventRecord(e0, st_), "event");
+                for (int r = 0; r < 2; ++r)
+                    check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
+                check(hipEventRecord(e1, st_), "event");
+                check(hipEventSynchronize(e1), "event");
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, e0, e1), "event");
+                score[c] = std::min(score[c], ms);
             }
         }
+        float bestMs = 1e30f;
+        for (int c = 0; c < kCands; ++c)
+            if (score[c] < bestMs) {
+                bestMs = score[c];
+                best = {cands[c][0], cands[c][1]};
+            }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         std::lock_guard<std::mutex> g(g_tuneMutex);
