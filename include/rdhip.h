@@ -199,6 +199,13 @@ int rdhip_evolve_fixed(rdhip_ctx* ctx, double r, int64_t iters);
 int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* counts, void* sums,
                            uint16_t* moved, int* out_flag);
 
+/* Chain ordering of the current chosen points (e.g. after rdhip_ridge_detection), the
+ * reference CPU class's `order_` step: rd::orderSamples (rd/cpu/brute_force/order.hpp:42-142)
+ * with the condition |p - q|^2 <= r2^2 of rd/cpu/brute_force/ridge_detection.inl:148-168.
+ * order (capacity K) receives point indices (rdhip_get_points order) chain after chain;
+ * chain_offsets (capacity K + 1) the chain boundaries; out_chains the number of chains. */
+int rdhip_order_chains(rdhip_ctx* ctx, double r2, int32_t* order, int64_t* chain_offsets, int64_t* out_chains);
+
 /* ---- differentiated evolve (no reference counterpart: the reference has no derivative) ----
  *
  * rdhip_evolve_fwd runs exactly `iters` evolve iterations with radius r from the current

@@ -123,6 +123,17 @@ class Oracle:
                                               self._scalar(P.dtype, r))
         return S, int(it)
 
+    def order(self, S, r2):
+        """Chain ordering of a ridge (rd::orderSamples with the r2 condition): returns
+        (order, offsets) -- point indices chain after chain, and the chain boundaries."""
+        S = np.ascontiguousarray(S)
+        K, d = S.shape
+        order = np.empty(max(K, 1), dtype=np.int64)
+        offsets = np.empty(K + 1, dtype=np.int64)
+        nc = self._check(self._fn("order", S.dtype)(_p(S), C.c_long(K), C.c_int(d), self._scalar(S.dtype, r2),
+                                                     _p(order), _p(offsets)), "order")
+        return order[:K].copy(), offsets[:nc + 1].copy()
+
     # -- differentiated evolve (frozen memberships; restatement only) ---------------
     def evolve_tape(self, P, S, r, iters):
         """`iters` evolve iterations from S recording what the frozen map needs.

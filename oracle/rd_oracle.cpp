@@ -12,11 +12,13 @@
 // Pinning (what checks each function, beyond retyping the formula):
 //   * golden fixtures produced by the compiled reference headers (oracle/ref_driver.cpp,
 //     tests/golden/make_golden.py) for choose, one evolve iteration, full evolve,
-//     decimate, brute-force ridge detection, the three tile trees and the five tiled
-//     drivers (tests/test_oracle_golden.py);
+//     decimate, brute-force ridge detection, chain ordering, the three tile trees and the
+//     tiled drivers (tests/test_oracle_golden.py);
 //   * mathematical invariants (greedy cover / separation for choose, brute-force
-//     argmin for the evolve assignment, partition invariants for the trees)
-//     in tests/test_oracle_invariants.py.
+//     argmin for the evolve assignment, partition invariants for the trees, chain
+//     adjacency and coverage for the ordering) in tests/test_oracle_invariants.py;
+//   * the differentiated evolve (no reference counterpart) by the bit-exact replay of
+//     recorded runs and central finite differences (tests/test_oracle_vjp.py).
 //
 // Conventions: points are ROW_MAJOR (point i occupies P[i*dim .. i*dim+dim-1]).
 
@@ -24,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <memory>
 #include <numeric>
@@ -669,6 +672,48 @@ static long tiledRD(int alg, const T* P, long n, int dim, T r1, T r2, size_t cap
 }
 
 // ---------------------------------------------------------------------------------
+// Chain ordering of a ridge: rd/cpu/brute_force/order.hpp:42-142 with the condition of
+// ridge_detection.inl:148-168 (squared distance <= r2*r2).  Every point still unchained
+// seeds a chain, in input order; the chain then repeatedly takes the first unchained point
+// after the seed (input order) lying within r2 of its front -- tested first -- or of its
+// back, and adds it at that end, until no point qualifies.  Chains are written one after
+// another into `order` (point indices); offsets[c] .. offsets[c+1] delimit chain c.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static long orderChains(const T* S, long K, int dim, T r2, long* order, long* offsets) {
+    const T r2Sqr = r2 * r2;
+    std::vector<char> unchained((size_t)std::max<long>(K, 1), 1);
+    long nc = 0, w = 0;
+    for (long seed = 0; seed < K; ++seed) {
+        if (!unchained[seed]) continue;
+        std::deque<long> chain{seed};
+        for (;;) {
+            long pick = -1;
+            bool atFront = false;
+            for (long j = seed + 1; j < K && pick < 0; ++j) {
+                if (!unchained[j]) continue;
+                if (sqdist(S + j * dim, S + chain.front() * dim, dim) <= r2Sqr) {
+                    pick = j;
+                    atFront = true;
+                } else if (sqdist(S + j * dim, S + chain.back() * dim, dim) <= r2Sqr) {
+                    pick = j;
+                }
+            }
+            if (pick < 0) break;
+            unchained[pick] = 0;
+            if (atFront)
+                chain.push_front(pick);
+            else
+                chain.push_back(pick);
+        }
+        offsets[nc++] = w;
+        for (long v : chain) order[w++] = v;
+    }
+    offsets[nc] = w;
+    return nc;
+}
+
+// ---------------------------------------------------------------------------------
 // Differentiated evolve with frozen memberships.  The reference has no derivative; these
 // three functions define the differentiable map the GPU forward/backward pair computes.
 // With the labels l_t(n) and the moved flags m_t(k,d) of a recorded run held fixed,
@@ -940,6 +985,9 @@ int oracle_num_threads() {
             oracle::evolveVjp<T>(n, K, dim, iters, labels, counts, moved, G, gradP);                 \
             return 0;                                                                                \
         });                                                                                          \
+    }                                                                                                \
+    long oracle_order_##SUF(const T* S, long K, int dim, T r2, long* order, long* offsets) {         \
+        return guarded([&]() -> long { return oracle::orderChains<T>(S, K, dim, r2, order, offsets); }); \
     }                                                                                                \
     long oracle_tree_##SUF(int kind, const T* P, long n, int dim, T r1, long cap, const long* tpd,   \
                            long* leafInfo, long infoCap, long* ownIdx, long ownCap, long* nbIdx,     \

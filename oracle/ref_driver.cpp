@@ -14,6 +14,7 @@
 #include "rd/cpu/brute_force/choose.hpp"
 #include "rd/cpu/brute_force/evolve.hpp"
 #include "rd/cpu/brute_force/decimate.hpp"
+#include "rd/cpu/brute_force/order.hpp"
 #include "rd/cpu/tiled/rd_tiled.hpp"
 #include "rd/cpu/tiled/local_tile_tree.hpp"
 #include "rd/cpu/tiled/grouped_tile_tree.hpp"
@@ -265,6 +266,23 @@ long refTree(int kind, const T* P, long n, int dim, T r1, long cap, const long* 
 
 }  // namespace
 
+// rd::orderSamples (rd/cpu/brute_force/order.hpp:42-142) with the condition of
+// ridge_detection.inl:148-168; chains as point indices (pointer - S) / dim.
+template <typename T>
+long refOrder(const T* S, long K, int dim, T r2, long* order, long* offsets) {
+    const T r2Sqr = r2 * r2;
+    auto chains = rd::orderSamples<T>(S, (size_t)dim, (size_t)K, [&r2Sqr](T const* p1, T const* p2, size_t d) -> bool {
+        return rd::squareEuclideanDistance(p1, p2, d) <= r2Sqr;
+    });
+    long nc = 0, w = 0;
+    for (auto& c : chains) {
+        offsets[nc++] = w;
+        for (T const* p : c) order[w++] = (long)((p - S) / dim);
+    }
+    offsets[nc] = w;
+    return nc;
+}
+
 extern "C" {
 
 int ref_openmp_threads() {
@@ -295,6 +313,9 @@ int ref_openmp_threads() {
     long ref_decimate_nocopy_##SUF(T* S, long K, int dim, T r2, int nLists, const long* sizes,       \
                                    unsigned char* alive) {                                           \
         return refDecimateNoCopy<T>(S, K, dim, r2, nLists, sizes, alive);                            \
+    }                                                                                                \
+    long ref_order_##SUF(const T* S, long K, int dim, T r2, long* order, long* offsets) {            \
+        return refOrder<T>(S, K, dim, r2, order, offsets);                                           \
     }                                                                                                \
     long ref_rd_##SUF(const T* P, long n, int dim, T r1, T r2, T* S, int omp) {                      \
         return refRD<T>(P, n, dim, r1, r2, S, omp);                                                  \

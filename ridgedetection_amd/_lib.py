@@ -30,7 +30,7 @@ EXPORTS = [
     "rdhip_ridge_detection", "rdhip_tiled_ridge_detection", "rdhip_get_count", "rdhip_get_points",
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
-    "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd",
+    "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
@@ -80,6 +80,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
         "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
         "rdhip_evolve_fwd": (i32, [vp, C.c_double, i64, i64]),
+        "rdhip_order_chains": (i32, [vp, C.c_double, vp, vp, C.POINTER(i64)]),
         "rdhip_evolve_bwd": (i32, [vp, vp, vp, vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
@@ -239,6 +240,15 @@ class Context:
 
     def evolve_fixed(self, r, iters):
         self._check(self.lib.rdhip_evolve_fixed(self.h, float(r), int(iters)))
+
+    def order_chains(self, r2):
+        """Chain ordering of the current chosen points: (order, offsets) as the oracle's."""
+        K = self.count()
+        order = np.empty(max(K, 1), dtype=np.int32)
+        off = np.empty(K + 1, dtype=np.int64)
+        nc = C.c_int64()
+        self._check(self.lib.rdhip_order_chains(self.h, float(r2), _ptr(order), _ptr(off), C.byref(nc)))
+        return order[:K].astype(np.int64), off[:nc.value + 1].copy()
 
     def evolve_fwd(self, r, iters, checkpoint_every=0):
         """Exactly `iters` evolve iterations, keeping what evolve_bwd needs."""

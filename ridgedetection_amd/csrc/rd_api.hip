@@ -485,6 +485,23 @@ int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* c
     });
 }
 
+int rdhip_order_chains(rdhip_ctx* ctx, double r2, int32_t* order, int64_t* chain_offsets, int64_t* out_chains) {
+    if (!(r2 >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points");
+        std::vector<int> ord;
+        std::vector<int64_t> off;
+        const int64_t nc = e.orderChains((T)r2, ord, off);
+        if (order)
+            for (size_t i = 0; i < ord.size(); ++i) order[i] = ord[i];
+        if (chain_offsets)
+            for (size_t i = 0; i < off.size(); ++i) chain_offsets[i] = off[i];
+        if (out_chains) *out_chains = nc;
+        return RDHIP_OK;
+    });
+}
+
 int rdhip_evolve_fwd(rdhip_ctx* ctx, double r, int64_t iters, int64_t checkpoint_every) {
     if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
     if (iters < 1) return fail(RDHIP_ERR_INVALID, "iteration count must be positive");

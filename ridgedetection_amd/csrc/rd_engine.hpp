@@ -157,6 +157,11 @@ public:
                         bool seedToPoints);
     bool chosenFromPoints() const { return chosenFromPoints_; }
 
+    // Chain ordering of the current chosen points (all segments, in getChosen order) with
+    // the reference's r2 condition: point indices chain after chain into `order`, chain
+    // boundaries into `offsets` (size chains + 1).  Returns the number of chains.
+    int64_t orderChains(T r2, std::vector<int>& order, std::vector<int64_t>& offsets);
+
     // -- drivers -------------------------------------------------------------------
     // Brute force over every segment independently (each its own group): choose, then
     // rounds of evolve + decimate until each segment's count is stable.

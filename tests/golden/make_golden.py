@@ -36,6 +36,9 @@ def brute_case(ref, name, P, r1, r2):
     d["decimate_S"], d["decimate_kept"] = SD, kept.astype(np.int32)
     SR, _ = ref.ridge_detection(P, r1, r2)
     d["rd_S"] = SR
+    # chain ordering of the ridge (r2 condition) and of the raw chosen set (r1)
+    d["order_idx"], d["order_off"] = ref.order(SR, r2)
+    d["order0_idx"], d["order0_off"] = ref.order(S0, r1)
     np.savez_compressed(os.path.join(OUT, f"brute_{name}.npz"), **d)
     print(f"brute_{name}: n={len(P)} K0={len(S0)} K={len(SR)}")
 

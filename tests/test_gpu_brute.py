@@ -134,3 +134,22 @@ def test_scan_shapes_and_autotuner(oracle):
         np.testing.assert_array_equal(c.ridge_detection(0.2, 0.4), ref)
         t = c.timers()
         assert t["scan_ppt"] in (1, 2, 4) and t["scan_threads"] in (128, 256, 512)
+
+
+@pytest.mark.parametrize("path", BRUTE, ids=_name)
+def test_order_chains_match_golden(path):
+    """Chain ordering (rd/cpu/brute_force/order.hpp) of the ridge and of the chosen set,
+    against the reference's chains stored in the fixtures."""
+    g = np.load(path)
+    P, r1, r2 = g["P"], float(g["r1"]), float(g["r2"])
+    _say(f"case order {_name(path)}")
+    with _ctx(P) as c:
+        c.choose(r1)
+        idx, off = c.order_chains(r1)
+        np.testing.assert_array_equal(idx, g["order0_idx"])
+        np.testing.assert_array_equal(off, g["order0_off"])
+        S = c.ridge_detection(r1, r2)
+        np.testing.assert_array_equal(S, g["rd_S"])
+        idx, off = c.order_chains(r2)
+        np.testing.assert_array_equal(idx, g["order_idx"])
+        np.testing.assert_array_equal(off, g["order_off"])

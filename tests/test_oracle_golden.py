@@ -37,6 +37,11 @@ def test_brute_phases(oracle, path):
     np.testing.assert_array_equal(kept, g["decimate_kept"])
     SR, _ = oracle.ridge_detection(P, r1, r2)
     np.testing.assert_array_equal(SR, g["rd_S"])
+    # chain ordering (rd/cpu/brute_force/order.hpp) of the ridge and of the chosen set
+    for S, r, key in ((SR, r2, "order"), (S0, r1, "order0")):
+        idx, off = oracle.order(S, r)
+        np.testing.assert_array_equal(idx, g[f"{key}_idx"])
+        np.testing.assert_array_equal(off, g[f"{key}_off"])
 
 
 @pytest.mark.parametrize("path", NOCOPY, ids=_name)

@@ -107,3 +107,30 @@ def test_tree_partition(oracle, kind):
     else:                                                   # plus points on the global max
         on_max = np.any(P == P.max(0), axis=1)
         assert len(np.setdiff1d(missing, np.nonzero(on_max)[0])) <= 2 * 27
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_order_chains_invariants(oracle, seed):
+    """Chain ordering: every point in exactly one chain; consecutive points of a chain
+    within r; a chain's smallest index is its seed, and seeds increase from chain to
+    chain; a chain cannot be extended by any point of a later chain that lies after its
+    seed (the first-fit walk stopped only when nothing qualified)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    S = g.uniform(-1, 1, (300, 2)).astype(np.float64)
+    r = 0.12
+    idx, off = oracle.order(S, r)
+    assert sorted(idx.tolist()) == list(range(len(S)))
+    seeds = []
+    for c in range(len(off) - 1):
+        ch = idx[off[c]:off[c + 1]]
+        for a, b in zip(ch[:-1], ch[1:]):
+            assert ((S[a] - S[b]) ** 2).sum() <= r * r
+        seeds.append(ch.min())
+    assert seeds == sorted(seeds)
+    for c in range(len(off) - 1):
+        ch = idx[off[c]:off[c + 1]]
+        later = idx[off[c + 1]:]
+        for e in (ch[0], ch[-1]):
+            for j in later:
+                if j > seeds[c]:
+                    assert ((S[j] - S[e]) ** 2).sum() > r * r
