@@ -199,6 +199,15 @@ int rdhip_evolve_fixed(rdhip_ctx* ctx, double r, int64_t iters);
 int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* counts, void* sums,
                            uint16_t* moved, int* out_flag);
 
+/* Quality against an ideal pattern (rd::RDAssessmentQuality, rd/utils/assessment_quality.hpp:
+ * 91-220).  A and B are host ROW_MAJOR point sets of the context's value type and dimension.
+ *   rdhip_set_distances: out[i] (value type) = distance from A[i] to the nearest point of B;
+ *   rdhip_hausdorff: max of the two directed set distances, -1 for an empty A;
+ *   rdhip_distance_stats: out4 = (median, average, min, max) of the distances from A to B. */
+int rdhip_set_distances(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, void* out);
+int rdhip_hausdorff(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, double* out);
+int rdhip_distance_stats(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, double* out4);
+
 /* Chain ordering of the current chosen points (e.g. after rdhip_ridge_detection), the
  * reference CPU class's `order_` step: rd::orderSamples (rd/cpu/brute_force/order.hpp:42-142)
  * with the condition |p - q|^2 <= r2^2 of rd/cpu/brute_force/ridge_detection.inl:148-168.

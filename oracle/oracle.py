@@ -134,6 +134,32 @@ class Oracle:
                                                      _p(order), _p(offsets)), "order")
         return order[:K].copy(), offsets[:nc + 1].copy()
 
+    # -- quality against an ideal pattern (rd/utils/assessment_quality.hpp) -----------
+    def hausdorff(self, A, B):
+        A = np.ascontiguousarray(A)
+        B = np.ascontiguousarray(B, dtype=A.dtype)
+        out = np.zeros(1, dtype=A.dtype)
+        self._check(self._fn("hausdorff", A.dtype)(_p(A), C.c_long(len(A)), _p(B), C.c_long(len(B)),
+                                                    C.c_int(A.shape[1]), _p(out)), "hausdorff")
+        return out[0]
+
+    def distance_stats(self, A, B):
+        """(median, average, min, max) of the distances from every point of A to B."""
+        A = np.ascontiguousarray(A)
+        B = np.ascontiguousarray(B, dtype=A.dtype)
+        out = np.zeros(4, dtype=A.dtype)
+        self._check(self._fn("distance_stats", A.dtype)(_p(A), C.c_long(len(A)), _p(B), C.c_long(len(B)),
+                                                         C.c_int(A.shape[1]), _p(out)), "distance_stats")
+        return out
+
+    def set_distances(self, A, B):
+        A = np.ascontiguousarray(A)
+        B = np.ascontiguousarray(B, dtype=A.dtype)
+        out = np.zeros(len(A), dtype=A.dtype)
+        self._check(self._fn("set_distances", A.dtype)(_p(A), C.c_long(len(A)), _p(B), C.c_long(len(B)),
+                                                        C.c_int(A.shape[1]), _p(out)), "set_distances")
+        return out
+
     # -- differentiated evolve (frozen memberships; restatement only) ---------------
     def evolve_tape(self, P, S, r, iters):
         """`iters` evolve iterations from S recording what the frozen map needs.

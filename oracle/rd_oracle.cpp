@@ -714,6 +714,49 @@ static long orderChains(const T* S, long K, int dim, T r2, long* order, long* of
 }
 
 // ---------------------------------------------------------------------------------
+// Quality against an ideal pattern: rd/utils/assessment_quality.hpp:91-220.  distance(a, B)
+// = sqrt(min_b |a - b|^2) (strict < from max(), the reference's order of operations);
+// setDistance(A, B) = the largest of those; Hausdorff = max(setDistance(A, B),
+// setDistance(B, A)) (-1 for an empty A); stats = (median = sorted[n/2], average summed in
+// sorted order, min, max) of the distances from A to B.
+// ---------------------------------------------------------------------------------
+template <typename T>
+static void setDistances(const T* A, long nA, const T* B, long nB, int dim, T* out) {
+    for (long i = 0; i < nA; ++i) {
+        T best = std::numeric_limits<T>::max();
+        for (long j = 0; j < nB; ++j) {
+            const T d = sqdist(A + i * dim, B + j * dim, dim);
+            if (d < best) best = d;
+        }
+        out[i] = std::sqrt(best);
+    }
+}
+
+template <typename T>
+static T hausdorff(const T* A, long nA, const T* B, long nB, int dim) {
+    if (nA == 0) return T(-1);
+    std::vector<T> da((size_t)nA), db((size_t)std::max<long>(nB, 1));
+    setDistances(A, nA, B, nB, dim, da.data());
+    setDistances(B, nB, A, nA, dim, db.data());
+    const T ab = *std::max_element(da.begin(), da.end());
+    const T ba = nB > 0 ? *std::max_element(db.begin(), db.begin() + nB) : -std::numeric_limits<T>::max();
+    return std::max(ab, ba);
+}
+
+template <typename T>
+static void distanceStats(const T* A, long nA, const T* B, long nB, int dim, T* out4) {
+    std::vector<T> d((size_t)nA);
+    setDistances(A, nA, B, nB, dim, d.data());
+    std::sort(d.begin(), d.end());
+    out4[2] = d.front();
+    out4[3] = d.back();
+    out4[0] = d[d.size() / 2];
+    T avg = 0;
+    for (T v : d) avg += v;
+    out4[1] = avg / (T)d.size();
+}
+
+// ---------------------------------------------------------------------------------
 // Differentiated evolve with frozen memberships.  The reference has no derivative; these
 // three functions define the differentiable map the GPU forward/backward pair computes.
 // With the labels l_t(n) and the moved flags m_t(k,d) of a recorded run held fixed,
@@ -983,6 +1026,19 @@ int oracle_num_threads() {
                                  const int* counts, const unsigned char* moved, T* G, T* gradP) {    \
         return guarded([&]() -> long {                                                               \
             oracle::evolveVjp<T>(n, K, dim, iters, labels, counts, moved, G, gradP);                 \
+            return 0;                                                                                \
+        });                                                                                          \
+    }                                                                                                \
+    long oracle_set_distances_##SUF(const T* A, long nA, const T* B, long nB, int dim, T* out) {    \
+        return guarded([&]() -> long { oracle::setDistances<T>(A, nA, B, nB, dim, out); return 0; }); \
+    }                                                                                                \
+    long oracle_hausdorff_##SUF(const T* A, long nA, const T* B, long nB, int dim, T* out) {         \
+        return guarded([&]() -> long { *out = oracle::hausdorff<T>(A, nA, B, nB, dim); return 0; });  \
+    }                                                                                                \
+    long oracle_distance_stats_##SUF(const T* A, long nA, const T* B, long nB, int dim, T* out4) {  \
+        return guarded([&]() -> long {                                                               \
+            if (nA < 1) throw std::runtime_error("distance stats of an empty set");                 \
+            oracle::distanceStats<T>(A, nA, B, nB, dim, out4);                                       \
             return 0;                                                                                \
         });                                                                                          \
     }                                                                                                \

@@ -15,6 +15,7 @@
 #include "rd/cpu/brute_force/evolve.hpp"
 #include "rd/cpu/brute_force/decimate.hpp"
 #include "rd/cpu/brute_force/order.hpp"
+#include "rd/utils/assessment_quality.hpp"
 #include "rd/cpu/tiled/rd_tiled.hpp"
 #include "rd/cpu/tiled/local_tile_tree.hpp"
 #include "rd/cpu/tiled/grouped_tile_tree.hpp"
@@ -283,6 +284,28 @@ long refOrder(const T* S, long K, int dim, T r2, long* order, long* offsets) {
     return nc;
 }
 
+// rd::RDAssessmentQuality (rd/utils/assessment_quality.hpp:91-220) with B as the ideal pattern.
+template <typename T>
+struct RefQuality : rd::RDAssessmentQuality<T> {
+    RefQuality(const T* B, long nB, int dim) : rd::RDAssessmentQuality<T>((size_t)nB, (size_t)dim) {
+        std::copy(B, B + nB * dim, this->idealPattern.begin());
+    }
+};
+
+template <typename T>
+long refHausdorff(const T* A, long nA, const T* B, long nB, int dim, T* out) {
+    RefQuality<T> q(B, nB, dim);
+    *out = q.hausdorffDistance(std::vector<T>(A, A + nA * dim));
+    return 0;
+}
+
+template <typename T>
+long refDistanceStats(const T* A, long nA, const T* B, long nB, int dim, T* out4) {
+    RefQuality<T> q(B, nB, dim);
+    q.setDistanceStats(std::vector<T>(A, A + nA * dim), out4[0], out4[1], out4[2], out4[3]);
+    return 0;
+}
+
 extern "C" {
 
 int ref_openmp_threads() {
@@ -313,6 +336,12 @@ int ref_openmp_threads() {
     long ref_decimate_nocopy_##SUF(T* S, long K, int dim, T r2, int nLists, const long* sizes,       \
                                    unsigned char* alive) {                                           \
         return refDecimateNoCopy<T>(S, K, dim, r2, nLists, sizes, alive);                            \
+    }                                                                                                \
+    long ref_hausdorff_##SUF(const T* A, long nA, const T* B, long nB, int dim, T* out) {            \
+        return refHausdorff<T>(A, nA, B, nB, dim, out);                                              \
+    }                                                                                                \
+    long ref_distance_stats_##SUF(const T* A, long nA, const T* B, long nB, int dim, T* out4) {     \
+        return refDistanceStats<T>(A, nA, B, nB, dim, out4);                                         \
     }                                                                                                \
     long ref_order_##SUF(const T* S, long K, int dim, T r2, long* order, long* offsets) {            \
         return refOrder<T>(S, K, dim, r2, order, offsets);                                           \

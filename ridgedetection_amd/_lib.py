@@ -31,6 +31,7 @@ EXPORTS = [
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
+    "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
@@ -81,6 +82,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
         "rdhip_evolve_fwd": (i32, [vp, C.c_double, i64, i64]),
         "rdhip_order_chains": (i32, [vp, C.c_double, vp, vp, C.POINTER(i64)]),
+        "rdhip_set_distances": (i32, [vp, vp, i64, vp, i64, vp]),
+        "rdhip_hausdorff": (i32, [vp, vp, i64, vp, i64, C.POINTER(C.c_double)]),
+        "rdhip_distance_stats": (i32, [vp, vp, i64, vp, i64, vp]),
         "rdhip_evolve_bwd": (i32, [vp, vp, vp, vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
@@ -240,6 +244,27 @@ class Context:
 
     def evolve_fixed(self, r, iters):
         self._check(self.lib.rdhip_evolve_fixed(self.h, float(r), int(iters)))
+
+    def set_distances(self, A, B):
+        A = np.ascontiguousarray(A, dtype=self.dtype)
+        B = np.ascontiguousarray(B, dtype=self.dtype)
+        out = np.empty(len(A), dtype=self.dtype)
+        self._check(self.lib.rdhip_set_distances(self.h, _ptr(A), len(A), _ptr(B), len(B), _ptr(out)))
+        return out
+
+    def hausdorff(self, A, B):
+        A = np.ascontiguousarray(A, dtype=self.dtype)
+        B = np.ascontiguousarray(B, dtype=self.dtype)
+        out = C.c_double()
+        self._check(self.lib.rdhip_hausdorff(self.h, _ptr(A), len(A), _ptr(B), len(B), C.byref(out)))
+        return out.value
+
+    def distance_stats(self, A, B):
+        A = np.ascontiguousarray(A, dtype=self.dtype)
+        B = np.ascontiguousarray(B, dtype=self.dtype)
+        out = np.zeros(4, dtype=np.float64)
+        self._check(self.lib.rdhip_distance_stats(self.h, _ptr(A), len(A), _ptr(B), len(B), _ptr(out)))
+        return out
 
     def order_chains(self, r2):
         """Chain ordering of the current chosen points: (order, offsets) as the oracle's."""

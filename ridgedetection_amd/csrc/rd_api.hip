@@ -1,6 +1,7 @@
 // rd_api.hip -- the C ABI (include/rdhip.h) over the device engine.
 #include "../../include/rdhip.h"
 #include "rd_engine.hpp"
+#include "rd_quality.hpp"
 #include "rd_tiled.hpp"
 #include "rd_tree.hpp"
 
@@ -481,6 +482,38 @@ int rdhip_evolve_iteration(rdhip_ctx* ctx, double r, int32_t* labels, int32_t* c
         if (moved)
             for (int64_t k = 0; k < K; ++k) moved[k] = mv[k];
         if (out_flag) *out_flag = flag;
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_set_distances(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, void* out) {
+    if (nA < 0 || nB < 0 || (nA > 0 && (!A || !out)) || (nB > 0 && !B)) return fail(RDHIP_ERR_INVALID, "bad point sets");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        std::vector<T> d = set_distances<T>(static_cast<const T*>(A), nA, static_cast<const T*>(B), nB, ctx->dim,
+                                            e.stream());
+        std::copy(d.begin(), d.end(), static_cast<T*>(out));
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_hausdorff(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, double* out) {
+    if (nA < 0 || nB < 0 || (nA > 0 && !A) || (nB > 0 && !B) || !out) return fail(RDHIP_ERR_INVALID, "bad point sets");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        *out = (double)hausdorff_distance<T>(static_cast<const T*>(A), nA, static_cast<const T*>(B), nB, ctx->dim,
+                                             e.stream());
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_distance_stats(rdhip_ctx* ctx, const void* A, int64_t nA, const void* B, int64_t nB, double* out4) {
+    if (nA < 1 || nB < 0 || !A || (nB > 0 && !B) || !out4) return fail(RDHIP_ERR_INVALID, "bad point sets");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        T s[4];
+        distance_stats<T>(static_cast<const T*>(A), nA, static_cast<const T*>(B), nB, ctx->dim, e.stream(), s);
+        for (int i = 0; i < 4; ++i) out4[i] = (double)s[i];
         return RDHIP_OK;
     });
 }

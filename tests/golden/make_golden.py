@@ -39,6 +39,11 @@ def brute_case(ref, name, P, r1, r2):
     # chain ordering of the ridge (r2 condition) and of the raw chosen set (r1)
     d["order_idx"], d["order_off"] = ref.order(SR, r2)
     d["order0_idx"], d["order0_off"] = ref.order(S0, r1)
+    # quality of the ridge against a pattern (every 7th sample stands in for the ideal curve)
+    ideal = np.ascontiguousarray(P[::7])
+    d["quality_hausdorff"] = np.array(ref.hausdorff(SR, ideal), P.dtype) if len(SR) else np.array(-1, P.dtype)
+    if len(SR):
+        d["quality_stats"] = ref.distance_stats(SR, ideal)
     np.savez_compressed(os.path.join(OUT, f"brute_{name}.npz"), **d)
     print(f"brute_{name}: n={len(P)} K0={len(S0)} K={len(SR)}")
 

@@ -153,3 +153,18 @@ def test_order_chains_match_golden(path):
         idx, off = c.order_chains(r2)
         np.testing.assert_array_equal(idx, g["order_idx"])
         np.testing.assert_array_equal(off, g["order_off"])
+
+
+@pytest.mark.parametrize("path", BRUTE, ids=_name)
+def test_quality_matches_golden(path):
+    """Hausdorff distance and distance statistics against a pattern
+    (rd/utils/assessment_quality.hpp), against the reference's values in the fixtures."""
+    g = np.load(path)
+    if "quality_stats" not in g.files:
+        pytest.skip("empty ridge")
+    P, S = g["P"], g["rd_S"]
+    ideal = np.ascontiguousarray(P[::7])
+    _say(f"case quality {_name(path)}")
+    with _ctx(P) as c:
+        assert c.hausdorff(S, ideal) == float(g["quality_hausdorff"])
+        np.testing.assert_array_equal(c.distance_stats(S, ideal), g["quality_stats"].astype(np.float64))

@@ -37,6 +37,11 @@ def test_brute_phases(oracle, path):
     np.testing.assert_array_equal(kept, g["decimate_kept"])
     SR, _ = oracle.ridge_detection(P, r1, r2)
     np.testing.assert_array_equal(SR, g["rd_S"])
+    # quality against a pattern (rd/utils/assessment_quality.hpp)
+    ideal = np.ascontiguousarray(P[::7])
+    if len(SR):
+        assert oracle.hausdorff(SR, ideal) == g["quality_hausdorff"]
+        np.testing.assert_array_equal(oracle.distance_stats(SR, ideal), g["quality_stats"])
     # chain ordering (rd/cpu/brute_force/order.hpp) of the ridge and of the chosen set
     for S, r, key in ((SR, r2, "order"), (S0, r1, "order0")):
         idx, off = oracle.order(S, r)
