@@ -112,11 +112,16 @@ enum rdhip_option {
     RDHIP_OPT_LOCAL_MAX_WORK = 7,     /* limit on samples x chosen points (default 2^22) */
     RDHIP_OPT_SCAN_THREADS = 8,    /* threads per evolve-scan block: 128, 256 or 512 (setting it
                                       turns the autotuner off) */
-    RDHIP_OPT_AUTOTUNE = 9         /* 1 (default): a single cloud's evolve, on first use for a
+    RDHIP_OPT_AUTOTUNE = 9,        /* 1 (default): a single cloud's evolve, on first use for a
                                       (value type, dimension, log2 samples, log2 chosen points)
                                       class, times every (samples per lane x threads per block)
                                       scan shape on the data and keeps the fastest (cached per
                                       process); batches keep 4 x 256 */
+    RDHIP_OPT_SEGMENTED = 10       /* 1: a batch whose active segments all have at most 65536
+                                      samples and 1024 chosen points iterates as a batch-wide scan
+                                      plus one workgroup per segment for the member sort, ordered
+                                      sums and shift; 0 (default, faster on the measured tiled and
+                                      batched workloads): batch-wide sort and sums */
 };
 
 int rdhip_version(void);

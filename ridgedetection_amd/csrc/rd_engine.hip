@@ -442,11 +442,6 @@ void Engine<T>::choose() {
 // ------------------------------------------------------------------------------------
 template <typename T>
 void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
-    int* labels = log ? log->labels : labels_.get();
-    ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
-                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
-                   compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
-                   compact() ? flags_.get() + 4 : nullptr};
     const bool ev = options.profile && slot >= 0;
     if (ev) {
         while ((int)events_.size() < 3 * (slot + 1)) {
@@ -456,8 +451,27 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
         }
         check(hipEventRecord(events_[3 * slot], st_), "event");
     }
-    // one segment: a block per work item; batches of segments: persistent blocks on the
-    // active-item queue
+    if (segmentedNow_ && !log && !record) {
+        // batch-wide scan into labels_, then one workgroup per segment sorts and sums
+        ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(),
+                       dSegK_.get(), dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
+                       nullptr, nullptr, nullptr};
+        check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
+        if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
+        LocalArgs<T> la{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), dSegK_.get(),
+                        dSegR1Sqr_.get(), nullptr, (int)segs_.size(), segKStride_, segChunk_, 0, 0, nullptr, nullptr,
+                        nullptr, labels_.get(), dSegDone_.get(), dSegMoved_.get(), flags_.get()};
+        check(launch_segment_sums<T>(la, dim_, (int)segs_.size(), st_), "segment_sums");
+        if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
+        return;
+    }
+    int* labels = log ? log->labels : labels_.get();
+    ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+                   dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
+                   compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
+                   compact() ? flags_.get() + 4 : nullptr};
+    // one block per work item (batches: the active-item list, or persistent blocks on its
+    // queue when that form is selected)
     check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads,
                                 compact() ? persistentBlocks() : 0, st_),
           "evolve_scan");
@@ -673,6 +687,22 @@ bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGrap
 }
 
 template <typename T>
+bool Engine<T>::chooseSegmented(const std::vector<int>& done) {
+    if (!options.segmented || !compact()) return false;
+    const std::vector<int> K = countsHost();
+    int maxK = 0, maxP = 0;
+    for (size_t s = 0; s < segs_.size(); ++s) {
+        if (done[s]) continue;
+        if (segs_[s].nP > options.segmentedMaxP || K[s] > kLocalMaxK) return false;
+        maxK = std::max(maxK, K[s]);
+        maxP = std::max(maxP, (int)segs_[s].nP);
+    }
+    segKStride_ = (int)round_up(std::max(maxK, 1), kChosenTile);
+    segChunk_ = std::min(std::max(maxP, 1), kLocalChunk);
+    return local_lds_bytes(dim_, sizeof(T), segKStride_, segChunk_) <= kLocalLdsBudget;
+}
+
+template <typename T>
 int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
     const int ns = (int)segs_.size();
     std::vector<int> K = countsHost();
@@ -715,7 +745,8 @@ int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
     LocalArgs<T> a{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegR1Sqr_.get(),
                    dLocalList_.get(), (int)list.size(), (int)round_up(std::max(maxK, 1), kChosenTile),
                    std::min(std::max(maxP, 1), kLocalChunk), maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0,
-                   (int)kEvolveGuard, dSegIter_.get(), flags_.get() + 5, flags_.get() + 6};
+                   (int)kEvolveGuard, dSegIter_.get(), flags_.get() + 5, flags_.get() + 6,
+                   nullptr, nullptr, nullptr, nullptr};
     check(launch_local_evolve<T>(a, dim_, st_), "local_evolve");
     int res[2] = {0, 0};
     check(hipMemcpyAsync(res, flags_.get() + 5, sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "local flags");
@@ -757,7 +788,14 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
     autotuneScan();
-    if (compact()) rebuildActive(done);
+    segmentedNow_ = chooseSegmented(done);
+    if (segmentedNow_) {
+        // the scan runs over every item (done segments' items return at once)
+        check(hipMemcpyAsync(dItems_.get(), items_.data(), sizeof(WorkItem) * items_.size(), hipMemcpyHostToDevice, st_),
+              "items");
+    } else if (compact()) {
+        rebuildActive(done);
+    }
     const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
     long long iters = 0;
     const int batch = std::max(1, options.itersPerSync);
@@ -785,7 +823,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
             if (exec) (void)hipGraphExecDestroy(exec);
             throw std::runtime_error("evolve did not converge within the iteration guard");
         }
-        if (compact()) {
+        if (compact() && !segmentedNow_) {
             // shrink the active work list once half of it belongs to converged segments
             check(hipMemcpy(doneNow.data(), dSegDone_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "done");
             int64_t remaining = 0;
@@ -808,6 +846,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         addEvolveBytes(segIter);
     }
     iters = std::max<long long>(iters, localIters);
+    segmentedNow_ = false;
     lastEvolveIters = iters;
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -88,6 +88,8 @@ struct RunOptions {
     int localMaxP = 16384;          // ... when every active segment has at most this many samples
     int localSmallP = 2048;         // ... otherwise only those with at most this many
     int64_t localMaxWork = 1 << 22; // ... and samples x chosen points at most this
+    bool segmented = false;         // batches whose active segments all have <= segmentedMaxP
+    int segmentedMaxP = 65536;      // samples: batch-wide scan + per-segment sort and sums
 };
 
 template <typename T>
@@ -277,6 +279,11 @@ private:
     // Evolves the small active segments with the segment-local kernel and marks them done;
     // returns the largest iteration count among them (0 when none).
     int evolveLocal(long long maxIter, std::vector<int>& done);
+    // Whether the active segments of a batch iterate as batch-wide scan + per-segment sort
+    // and sums (sets the LDS layout of that kernel).
+    bool chooseSegmented(const std::vector<int>& done);
+    bool segmentedNow_ = false;
+    int segKStride_ = 16, segChunk_ = 1;
     DevBuf<int> dLocalList_;
 };
 

@@ -31,6 +31,12 @@ struct LocalArgs {
     int* segIter;           // iterations run, per segment
     int* maxIters;          // max over the listed segments (atomicMax)
     int* error;
+    // one-iteration form (launch_segment_sums): labels of every position from the
+    // batch-wide scan, per-segment done / moved flags, the batch's all-done flag
+    const int* labelsIn;
+    const int* segDone;
+    int* segMoved;
+    const int* allDone;
 };
 
 // LDS bytes the local evolve needs for chosen stride kStride and chunks of `chunk` samples.
@@ -41,5 +47,11 @@ size_t local_lds_bytes(int dim, size_t valueBytes, int kStride, int chunk);
 // labels, member lists, ordered sums and shift inside one workgroup.
 template <typename T>
 hipError_t launch_local_evolve(const LocalArgs<T>& a, int dim, hipStream_t st);
+
+// One evolve iteration for segments 0 .. numSegs-1 (one workgroup each, done segments
+// skipped) from the labels of the batch-wide scan: counting sort, ordered sums and the
+// shift of rd/cpu/brute_force/evolve.hpp:32-77, 129-179, in LDS.
+template <typename T>
+hipError_t launch_segment_sums(const LocalArgs<T>& a, int dim, int numSegs, hipStream_t st);
 
 }  // namespace rdhip

@@ -83,30 +83,33 @@ def test_fixed_iterations_graph_matches_eager():
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("local", [0, 1])
-def test_segment_local_and_batchwide_iterations_agree(oracle, local):
-    """Small segments evolve inside one workgroup each (every iteration in LDS) unless the
-    option is off; both forms must give the oracle's result, on a batch mixing clouds below
-    and above the segment-local size limit and on tiled runs."""
-    from ridgedetection_amd._lib import Context, OPT_LOCAL
+@pytest.mark.parametrize("local,segmented", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_segment_local_and_batchwide_iterations_agree(oracle, local, segmented):
+    """Small segments evolve inside one workgroup each (every iteration in LDS), batches of
+    small segments iterate as a batch-wide scan plus per-segment sort and sums, unless the
+    options are off; every form must give the oracle's result, on a batch mixing clouds
+    below and above the size limits and on tiled runs."""
+    from ridgedetection_amd._lib import Context, OPT_LOCAL, OPT_SEGMENTED
     g = np.random.Generator(np.random.PCG64(77))
     clouds = []
-    for i, n in enumerate([900, 5000, 16384, 16385, 30000, 2500]):
+    for i, n in enumerate([900, 5000, 16384, 16385, 30000, 2500, 70000]):
         P = synth.spiral2d(n, seed=300 + i, sigma=0.08) if i % 2 else synth.segment_chain(n, 2, seed=300 + i, sigma=0.03)
         ext = float(np.max(P.max(0) - P.min(0)))
         r1 = float(g.uniform(0.02, 0.05)) * ext
         clouds.append((P, r1, 2 * r1))
-    print(f"case segment-local={local} batch", flush=True)
+    print(f"case segment-local={local} segmented={segmented} batch", flush=True)
     with Context(2, np.float32) as c:
         c.set_option(OPT_LOCAL, local)
+        c.set_option(OPT_SEGMENTED, segmented)
         res = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
     for (P, r1, r2), S in zip(clouds, res):
         np.testing.assert_array_equal(S, oracle.ridge_detection(P, r1, r2)[0])
     P = synth.helix3d(30000, seed=31)
     for alg in (6, 1, 0):
-        print(f"case segment-local={local} tiled alg={alg}", flush=True)
+        print(f"case segment-local={local} segmented={segmented} tiled alg={alg}", flush=True)
         with Context(3, np.float32) as c:
             c.set_option(OPT_LOCAL, local)
+            c.set_option(OPT_SEGMENTED, segmented)
             c.set_points(P)
             S = c.tiled(alg, 0.2, 0.4, 3000, [4, 4, 4])
         np.testing.assert_array_equal(S, oracle.tiled(alg, P, 0.2, 0.4, 3000, [4, 4, 4]))
