@@ -71,9 +71,12 @@ def cpu_baseline(P, S, r, budget_s=24.0):
 
 def batched_vs_looped(count=512, loop_every=8, seed=7):
     """512 clouds (256 with d=2, 256 with d=3), sizes log-uniform in [2000, 500000], each
-    a noisy spiral / helix / segment chain with r1 = 4 sigma, r2 = 2 r1: batched (one
-    launch per phase for all clouds of a dimension) against a per-cloud loop over every
-    `loop_every`-th cloud.  Both forms start from host arrays (uploads included)."""
+    a noisy spiral / helix / segment chain with r1 = 4 sigma, r2 = 2 r1: one batch of all
+    clouds (one launch per phase; the 2-D clouds run zero-padded to 3 coordinates, which
+    leaves their results bit-identical) against a per-cloud loop over every
+    `loop_every`-th cloud, each in a context of its own dimension.  Both forms start from
+    host arrays (uploads included).  The batch split by dimension (two batches, one per
+    context dimension) is timed too."""
     from ridgedetection_amd import synth
     from ridgedetection_amd._lib import Context
     log(f"bench: generating {count} clouds")
@@ -82,19 +85,28 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
     by_dim = {}
     for i, (p, r1, r2) in enumerate(clouds):
         by_dim.setdefault(p.shape[1], []).append(i)
+    dmax = max(by_dim)
     ctxs = {d: Context(d, np.float32) for d in by_dim}
+
+    def run(ctx, idx):
+        res = ctx.batch([clouds[i][0] for i in idx], [clouds[i][1] for i in idx], [clouds[i][2] for i in idx])
+        return sum(len(r) for r in res)
+
     # warm-up (kernel loading, allocations) on a few small clouds of each dimension
     for d, idx in by_dim.items():
-        small = sorted(idx, key=lambda i: len(clouds[i][0]))[:4]
-        ctxs[d].batch([clouds[i][0] for i in small], [clouds[i][1] for i in small], [clouds[i][2] for i in small])
-    log("bench: batched run")
+        run(ctxs[d], sorted(idx, key=lambda i: len(clouds[i][0]))[:4])
+    run(ctxs[dmax], [sorted(idx, key=lambda i: len(clouds[i][0]))[0] for idx in by_dim.values()])
+    log("bench: batched run (one mixed-dimension batch)")
     t0 = time.perf_counter()
-    out_counts = 0
-    for d, idx in by_dim.items():
-        res = ctxs[d].batch([clouds[i][0] for i in idx], [clouds[i][1] for i in idx], [clouds[i][2] for i in idx])
-        out_counts += sum(len(r) for r in res)
+    out_counts = run(ctxs[dmax], list(range(count)))
     t_batch = time.perf_counter() - t0
     log(f"bench: batched {count} clouds in {t_batch:.2f} s")
+    t0 = time.perf_counter()
+    split_counts = sum(run(ctxs[d], idx) for d, idx in by_dim.items())
+    t_split = time.perf_counter() - t0
+    log(f"bench: batched by dimension in {t_split:.2f} s")
+    if split_counts != out_counts:
+        raise RuntimeError("mixed and per-dimension batches disagree")
     sample = list(range(0, count, loop_every))
     for d in ctxs:       # warm single-cloud paths
         i = by_dim[d][0]
@@ -116,7 +128,7 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
             "batched_clouds_per_s": batched_rate, "looped_clouds_per_s": loop_rate,
             "batched_over_looped": batched_rate / loop_rate,
             "looped_sample": f"every {loop_every}th cloud ({len(sample)} clouds, same size distribution)",
-            "batched_s": t_batch, "looped_s": t_loop}
+            "batched_s": t_batch, "batched_by_dimension_s": t_split, "looped_s": t_loop}
 
 
 def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):

@@ -151,6 +151,16 @@ int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* 
                                 const int64_t* sizes, const double* r1, const double* r2,
                                 int64_t max_evolve_iters, int64_t* out_counts);
 
+/* The same for clouds of different dimensions: cloud c has dims[c] <= the context's
+ * dimension values per point (host ROW_MAJOR, n x dims[c]) and runs as that many
+ * coordinates followed by zeros -- the zero coordinates add +0 to every distance and never
+ * move, so each cloud's result equals its own-dimension run bit for bit.  rdhip_get_points
+ * returns rows of the context's dimension; cloud c's points are the first dims[c] values
+ * of its rows. */
+int rdhip_batch_ridge_detection_dims(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds,
+                                     const int64_t* dims, const int64_t* sizes, const double* r1,
+                                     const double* r2, int64_t max_evolve_iters, int64_t* out_counts);
+
 /* Tiled ridge detection (tree build, per-tile choose/evolve, local or global decimation,
  * optional end-phase refinement: decimate + evolve over all points). tiles_per_dim has dim
  * entries; max_tile_capacity: a tile with at least this many own points is split. */

@@ -31,7 +31,7 @@ EXPORTS = [
     "rdhip_get_timers", "rdhip_choose", "rdhip_set_chosen", "rdhip_evolve", "rdhip_evolve_iteration",
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
-    "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats",
+    "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats", "rdhip_batch_ridge_detection_dims",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
@@ -80,6 +80,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_build_tree": (i32, [vp, i32, C.c_double, i64, vp, C.POINTER(i64)]),
         "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
         "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
+        "rdhip_batch_ridge_detection_dims": (i32, [vp, i64, vp, vp, vp, vp, vp, i64, vp]),
         "rdhip_evolve_fwd": (i32, [vp, C.c_double, i64, i64]),
         "rdhip_order_chains": (i32, [vp, C.c_double, vp, vp, C.POINTER(i64)]),
         "rdhip_set_distances": (i32, [vp, vp, i64, vp, i64, vp]),
@@ -179,16 +180,22 @@ class Context:
         m = len(arrs)
         ptrs = (C.c_void_p * m)(*[a.ctypes.data for a in arrs])
         sizes = np.array([len(a) for a in arrs], dtype=np.int64)
+        dims = np.array([a.shape[1] for a in arrs], dtype=np.int64)
         vr1 = np.ascontiguousarray(r1, dtype=np.float64).reshape(m)
         vr2 = np.ascontiguousarray(r2, dtype=np.float64).reshape(m)
         counts = np.zeros(m, dtype=np.int64)
-        self._check(self.lib.rdhip_batch_ridge_detection(self.h, m, ptrs, _ptr(sizes), _ptr(vr1), _ptr(vr2),
-                                                         int(max_evolve_iters), _ptr(counts)))
+        if np.all(dims == self.dim):
+            self._check(self.lib.rdhip_batch_ridge_detection(self.h, m, ptrs, _ptr(sizes), _ptr(vr1), _ptr(vr2),
+                                                             int(max_evolve_iters), _ptr(counts)))
+        else:   # clouds of lower dimensions run zero-padded to the context's
+            self._check(self.lib.rdhip_batch_ridge_detection_dims(self.h, m, ptrs, _ptr(dims), _ptr(sizes),
+                                                                  _ptr(vr1), _ptr(vr2), int(max_evolve_iters),
+                                                                  _ptr(counts)))
         self.n = int(sizes.sum())
         allp = self.points()
         out, o = [], 0
-        for k in counts:
-            out.append(allp[o:o + k].copy())
+        for k, dc in zip(counts, dims):
+            out.append(np.ascontiguousarray(allp[o:o + k, :dc]))
             o += k
         return out
 

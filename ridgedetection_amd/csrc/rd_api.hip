@@ -148,8 +148,14 @@ int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evol
     });
 }
 
-int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, const int64_t* sizes,
-                                const double* r1, const double* r2, int64_t max_evolve_iters, int64_t* out_counts) {
+namespace {
+
+// Batched ridge detection; dims (optional) gives every cloud's own dimension <= the
+// context's: its rows are padded with zero coordinates, which leave every distance (the
+// padded terms come last and add +0), every sum, shift and decision unchanged.
+int batch_rd(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, const int64_t* dims,
+             const int64_t* sizes, const double* r1, const double* r2, int64_t max_evolve_iters,
+             int64_t* out_counts) {
     if (num_clouds < 1 || num_clouds > (int64_t(1) << 24)) return fail(RDHIP_ERR_INVALID, "cloud count out of range");
     if (ctx && (!clouds || !sizes || !r1 || !r2)) return fail(RDHIP_ERR_INVALID, "null batch arrays");
     if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
@@ -160,16 +166,25 @@ int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* 
         for (int64_t c = 0; c < num_clouds; ++c) {
             if (sizes[c] < 1 || !clouds[c]) return fail(RDHIP_ERR_INVALID, "every cloud needs at least one point");
             if (!(r1[c] >= 0) || !(r2[c] >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
+            if (dims && (dims[c] < 1 || dims[c] > dim))
+                return fail(RDHIP_ERR_INVALID, "cloud dimensions must be in [1, context dimension]");
             total += sizes[c];
         }
         if (total >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "batch exceeds 2^31 points");
-        std::vector<T> all((size_t)total * dim);
+        std::vector<T> all((size_t)total * dim, T(0));
         std::vector<SegDesc> segs((size_t)num_clouds);
         std::vector<int64_t> caps((size_t)num_clouds);
         std::vector<T> vr1((size_t)num_clouds), vr2((size_t)num_clouds);
         int64_t off = 0;
         for (int64_t c = 0; c < num_clouds; ++c) {
-            std::memcpy(all.data() + off * dim, clouds[c], sizeof(T) * sizes[c] * dim);
+            const int dc = dims ? (int)dims[c] : dim;
+            const T* src = static_cast<const T*>(clouds[c]);
+            if (dc == dim) {
+                std::memcpy(all.data() + off * dim, src, sizeof(T) * sizes[c] * dim);
+            } else {
+                for (int64_t i = 0; i < sizes[c]; ++i)
+                    std::memcpy(all.data() + (off + i) * dim, src + i * dc, sizeof(T) * dc);
+            }
             SegDesc s{};
             s.pOff = off;
             s.nP = (int32_t)sizes[c];
@@ -195,6 +210,20 @@ int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* 
         }
         return RDHIP_OK;
     });
+}
+
+}  // namespace
+
+int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, const int64_t* sizes,
+                                const double* r1, const double* r2, int64_t max_evolve_iters, int64_t* out_counts) {
+    return batch_rd(ctx, num_clouds, clouds, nullptr, sizes, r1, r2, max_evolve_iters, out_counts);
+}
+
+int rdhip_batch_ridge_detection_dims(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds,
+                                     const int64_t* dims, const int64_t* sizes, const double* r1, const double* r2,
+                                     int64_t max_evolve_iters, int64_t* out_counts) {
+    if (!dims) return fail(RDHIP_ERR_INVALID, "null dimension array");
+    return batch_rd(ctx, num_clouds, clouds, dims, sizes, r1, r2, max_evolve_iters, out_counts);
 }
 
 int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double r2, int64_t max_tile_capacity,

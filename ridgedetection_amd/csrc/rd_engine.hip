@@ -78,7 +78,8 @@ __global__ void relayout_slots_kernel(const T* __restrict__ S, int64_t sStride, 
 template <typename T>
 Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
     if (dim < 1 || dim > kMaxDim) throw std::invalid_argument("dimension must be in [1, 12]");
-    flags_.reserve(8);   // [0] all done, [1] iterations, [2] overflow, [3] queue, [4] active items
+    flags_.reserve(8);   // [0] all done, [1] iterations, [2] overflow, [3] queue, [4] active items,
+                         // [5] [6] local evolve
     dN_.reserve(1);
     dM_.reserve(1);
 }
@@ -299,6 +300,7 @@ void Engine<T>::ensureEvolveScratch() {
     const int64_t slots = std::max<int64_t>(totalSlots_, 1);
     chainOff_.reserve(slots);
     chainEnd_.reserve(slots);
+    longList_.reserve(long_list_entries(totalP, kLongMinShift));
     counts_.reserve(slots);
     moved_.reserve((size_t)slots * dim_);
     sums_.reserve((size_t)sStride_ * dim_);
@@ -490,14 +492,16 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
           "sort");
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), kb, 0, flags_.get());
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainEnd_.get(), kb, 0, flags_.get());
-    check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), flags_.get(), st_),
+    const int longShift = long_chain_shift(dim_, totalP);
+    check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
+                              longShift, flags_.get(), st_),
           "chain_bounds");
     ShiftArgs<T> ha{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
                     log ? log->moved : (record ? moved_.get() : nullptr),
-                    log ? log->movedStride : totalSlots_};
+                    log ? log->movedStride : totalSlots_, longList_.get(), longShift};
     check(launch_chain_sum_shift<T>(ha, dim_, totalP, false, st_), "chain_sum_shift");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
 }

@@ -207,6 +207,7 @@ private:
     DevBuf<int> chosenIdx_;
     // evolve scratch
     DevBuf<int> labels_, keys_, members_, chainOff_, chainEnd_, counts_;
+    DevBuf<int> longList_;   // wave-summed chains (ShiftArgs::longList)
     DevBuf<T> sums_, sortedP_;
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();

@@ -18,6 +18,10 @@
 //   4. chain_sum_shift -- one 16-lane row per (coordinate, chosen point) folds the members
 //      into the running sum in sample order (the reference's serial summation, so sums are
 //      bit-identical), applies the shift rule and raises the segment's "moved" flag.
+//      Chains holding a sorted position that is a multiple of L (every chain of at least L
+//      members; L adapts so that at most kLongWaves waves are needed) go to one wave each
+//      instead, in the same launch, which adds at the latency of a plain dependent add
+//      rather than that of a DPP-fed one.
 //   5. evolve_finalize -- per-segment iteration count and convergence; global all-done.
 #include "rd_evolve.hpp"
 #include "rd_rowsum.hpp"
@@ -31,13 +35,17 @@ __global__ __launch_bounds__(256) void chain_bounds_kernel(const int* __restrict
                                                            const int64_t* __restrict__ mPtr,
                                                            int* __restrict__ chainOff,
                                                            int* __restrict__ chainEnd,
+                                                           int* __restrict__ longList, int longShift,
                                                            const int* __restrict__ allDone) {
     if (*allDone) return;
     const int64_t M = *mPtr;
+    const int64_t L = int64_t(1) << longShift;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < M; i += (int64_t)gridDim.x * 256) {
         int k = keys[i];
         if (i == 0 || keys[i - 1] != k) chainOff[k] = (int)i;
         if (i == M - 1 || keys[i + 1] != k) chainEnd[k] = (int)(i + 1);
+        // i = m L is the first multiple of L inside its chain iff i - L lies outside it
+        if (longList && (i & (L - 1)) == 0) longList[i >> longShift] = (i < L || keys[i - L] != k) ? k : -1;
     }
 }
 
@@ -71,35 +79,11 @@ __global__ __launch_bounds__(256) void gather_members_kernel(const T* __restrict
 // -0, so it is never -0 and adding +0 leaves it unchanged.  The shift of one coordinate
 // depends only on that coordinate's sum (rd/cpu/brute_force/evolve.hpp:129-179).
 // ---------------------------------------------------------------------------------
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
-    if (*a.allDone) return;
-    constexpr int kRow = 16;
-    const int t = threadIdx.x & (kRow - 1);
-    const int64_t row = (int64_t)blockIdx.x * (256 / kRow) + threadIdx.x / kRow;
-    const int d = (int)(row / max<int64_t>(a.totalSlots, 1));
-    const int64_t slot = row - (int64_t)d * a.totalSlots;
-    int seg = -1, off = 0, cnt = 0;
-    if (d < DIM) {
-        seg = a.slotSeg[slot];
-        if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
-            off = a.chainOff[slot];
-            cnt = a.chainEnd[slot] - off;
-        } else {
-            seg = -1;
-        }
-    }
-    // the wave steps through the longest of its four chains; DPP needs every lane present.
-    // Loads are unconditional (index clamped into the chain, or element 0 of the buffer for
-    // an empty one) so that no branch separates them from the folds.
-    const int steps = wave_row_max(cnt);
-    const T* __restrict__ src = a.sortedP + (int64_t)min(d, DIM - 1) * a.mStride + off;
-    const int last = max(cnt - 1, 0);
-    const T sum = row_ordered_sum<T>(steps, [&](int i) {
-        const T x = src[min(i, last)];
-        return i < cnt ? x : T(0);
-    });
-    if (seg < 0 || t != 0) return;
+// Mean of a chain and the shift rule of rd/cpu/brute_force/evolve.hpp:129-179 for one
+// coordinate; optional per-slot outputs.
+template <typename T>
+__device__ __forceinline__ void shift_coordinate(const ShiftArgs<T>& a, int d, int64_t slot, int seg, int cnt,
+                                                 T sum) {
     if (a.outCounts && d == 0) a.outCounts[slot] = cnt;
     if (a.outSums) a.outSums[d * a.sStride + slot] = sum;
     bool moved = false;
@@ -118,6 +102,141 @@ __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a) {
     }
     if (a.outMoved) a.outMoved[d * a.movedStride + slot] = moved ? 1 : 0;
     if (moved) a.segMoved[seg] = 1;
+}
+
+// 16-lane row path: one row per (coordinate, chosen slot) of chains not on the long list
+template <typename T, int DIM>
+__device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t blk) {
+    constexpr int kRow = 16;
+    const int t = threadIdx.x & (kRow - 1);
+    const int64_t row = blk * (256 / kRow) + threadIdx.x / kRow;
+    const int d = (int)(row / max<int64_t>(a.totalSlots, 1));
+    const int64_t slot = row - (int64_t)d * a.totalSlots;
+    int seg = -1, off = 0, cnt = 0;
+    if (d < DIM) {
+        seg = a.slotSeg[slot];
+        if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
+            off = a.chainOff[slot];
+            cnt = a.chainEnd[slot] - off;
+            // a chain holding a multiple of L is summed by a wave (chain_sum_wave)
+            const int64_t s0 = off, e1 = (int64_t)off + cnt - 1;
+            if (a.longList && cnt > 0 && (e1 >> a.longShift) >= ((s0 + (int64_t(1) << a.longShift) - 1) >> a.longShift)) {
+                seg = -1;
+                cnt = 0;
+            }
+        } else {
+            seg = -1;
+        }
+    }
+    // the wave steps through the longest of its four chains; DPP needs every lane present.
+    // Loads are unconditional (index clamped into the chain, or element 0 of the buffer for
+    // an empty one) so that no branch separates them from the folds.
+    const int steps = wave_row_max(cnt);
+    const T* __restrict__ src = a.sortedP + (int64_t)min(d, DIM - 1) * a.mStride + off;
+    const int last = max(cnt - 1, 0);
+    const T sum = row_ordered_sum<T>(steps, [&](int i) {
+        const T x = src[min(i, last)];
+        return i < cnt ? x : T(0);
+    });
+    if (seg < 0 || t != 0) return;
+    shift_coordinate(a, d, slot, seg, cnt, sum);
+}
+
+// ---------------------------------------------------------------------------------
+// 4c. ordered sums of the long chains: one wave per (coordinate, chosen point).  Tiles of
+// kLongTile members are loaded coalesced one tile before their turn (the loads of tile
+// t + 1 are in flight while tile t is summed), staged in LDS and added in member order by
+// every lane from broadcast LDS reads, so each member costs one dependent add.  Members
+// past the end of the chain are staged as +0 (see above).
+// ---------------------------------------------------------------------------------
+constexpr int kLongPer = 16;
+constexpr int kLongTile = kLongPer * kWave;
+
+template <typename T>
+struct WideOf;
+template <>
+struct WideOf<float> { using type = float4; static constexpr int n = 4; };
+template <>
+struct WideOf<double> { using type = double2; static constexpr int n = 2; };
+
+// orders a wave's LDS accesses (the tile is private to the wave)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_ordered_sum(const T* __restrict__ src, int cnt, T* tile) {
+    using W = typename WideOf<T>::type;
+    constexpr int kW = WideOf<T>::n;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nTiles = (cnt + kLongTile - 1) / kLongTile;
+    const int last = max(cnt - 1, 0);
+    T reg[kLongPer];
+    // index clamped into the chain and masked (tile t + 1 is loaded after tile t is staged;
+    // its loads are not waited for before tile t's sum)
+    auto load = [&](int t) {
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) {
+            const int i = t * kLongTile + c * kWave + lane;
+            const T x = src[min(i, last)];
+            reg[c] = i < cnt ? x : T(0);
+        }
+    };
+    load(0);
+    T sum = T(0);
+    for (int t = 0; t < nTiles; ++t) {
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) tile[c * kWave + lane] = reg[c];
+        wave_lds_sync();
+        load(t + 1);
+        const W* w = reinterpret_cast<const W*>(tile);
+#pragma unroll
+        for (int q = 0; q < kLongTile / kW; ++q) {
+            const W v = w[q];
+            if constexpr (kW == 4) {
+                sum = sum + v.x;
+                sum = sum + v.y;
+                sum = sum + v.z;
+                sum = sum + v.w;
+            } else {
+                sum = sum + v.x;
+                sum = sum + v.y;
+            }
+        }
+    }
+    return sum;
+}
+
+// wave path: one wave per (list entry, coordinate)
+template <typename T, int DIM>
+__device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wave, T* tile) {
+    const int64_t e = wave / DIM;
+    const int d = (int)(wave - e * DIM);
+    if ((e << a.longShift) >= *a.mPtr) return;
+    const int slot = a.longList[e];
+    if (slot < 0) return;
+    const int seg = a.slotSeg[slot];
+    if (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg]) return;
+    const int off = a.chainOff[slot];
+    const int cnt = a.chainEnd[slot] - off;
+    const T sum = wave_ordered_sum<T>(a.sortedP + (int64_t)d * a.mStride + off, cnt, tile);
+    if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
+}
+
+// blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
+    __shared__ __align__(16) T tiles[256 / kWave][kLongTile];
+    if (*a.allDone) return;
+    if ((int64_t)blockIdx.x < rowBlocks) {
+        chain_sum_rows<T, DIM>(a, blockIdx.x);
+    } else {
+        const int w = threadIdx.x / kWave;
+        chain_sum_wave<T, DIM>(a, ((int64_t)blockIdx.x - rowBlocks) * (256 / kWave) + w, tiles[w]);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -168,26 +287,30 @@ __global__ __launch_bounds__(256) void fill_sentinel_kernel(T* __restrict__ S, i
 // ---------------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, bool gather, hipStream_t st) {
-    const int64_t blocks = ((int64_t)dim * a.totalSlots + 15) / 16;
-    if (blocks <= 0) return hipSuccess;
+    const int64_t rowBlocks = ((int64_t)dim * a.totalSlots + 15) / 16;
+    if (rowBlocks <= 0) return hipSuccess;
+    const int64_t waves = a.longList ? (int64_t)dim * long_list_entries(maxM, a.longShift) : 0;
+    const int64_t blocks = rowBlocks + (waves + 256 / kWave - 1) / (256 / kWave);
     const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
         if (gather)
             hipLaunchKernelGGL((gather_members_kernel<T, DIM>), dim3(gblocks), dim3(256), 0, st, a.P, a.pStride,
                                a.pIndex, a.members, a.mPtr, a.sortedP, a.mStride, a.allDone);
-        hipLaunchKernelGGL((chain_sum_shift_kernel<T, DIM>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((chain_sum_shift_kernel<T, DIM>), dim3((unsigned)blocks), dim3(256), 0, st, a,
+                           rowBlocks);
         ok = true;
     });
     return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
 hipError_t launch_chain_bounds(const int* keys, const int64_t* dM, int64_t maxM, int* chainOff,
-                               int* chainEnd, const int* allDone, hipStream_t st) {
+                               int* chainEnd, int* longList, int longShift, const int* allDone,
+                               hipStream_t st) {
     int64_t blocks = std::min<int64_t>((maxM + 255) / 256, 8192);
     if (blocks <= 0) return hipSuccess;
     hipLaunchKernelGGL(chain_bounds_kernel, dim3((unsigned)blocks), dim3(256), 0, st, keys, dM, chainOff,
-                       chainEnd, allDone);
+                       chainEnd, longList, longShift, allDone);
     return hipGetLastError();
 }
 

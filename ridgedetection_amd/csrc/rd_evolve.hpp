@@ -56,7 +56,23 @@ struct ShiftArgs {
     T* outSums;             // optional: SoA coordinate sums per slot
     unsigned char* outMoved;   // optional: 1 where coordinate d of a slot moved, at [d * movedStride + slot]
     int64_t movedStride;
+    // chains summed by a whole wave instead of a 16-lane row: those holding a sorted
+    // position that is a multiple of L = 2^longShift (so every chain of at least L members);
+    // longList[m] is the chain whose first such position is m L, else -1 (chain_bounds)
+    const int* longList;
+    int longShift;
 };
+
+// L of the wave-summed chains for up to maxM sorted members: the smallest power of two
+// >= 2^kLongMinShift with at most kLongWaves (coordinate, list entry) waves.
+constexpr int kLongMinShift = 10;
+constexpr int kLongWaves = 2048;
+inline int long_chain_shift(int dim, int64_t maxM) {
+    int sh = kLongMinShift;
+    while ((int64_t)dim * ((maxM + (int64_t(1) << sh) - 1) >> sh) > kLongWaves) ++sh;
+    return sh;
+}
+inline int64_t long_list_entries(int64_t maxM, int shift) { return std::max<int64_t>((maxM >> shift) + 1, 1); }
 
 // persistentBlocks > 0: that many persistent blocks (of kScanThreads) pull the work items
 // from a device queue; 0: one block of `threads` (128, 256 or 512) per work item, each item
@@ -69,8 +85,10 @@ hipError_t launch_evolve_scan(const ScanArgs<T>& a, int dim, int numItems, int p
 // (pass the live bound, not the capacity).
 template <typename T>
 hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, bool gather, hipStream_t st);
+// longList (optional): the wave-summed chain list of ShiftArgs for L = 2^longShift
 hipError_t launch_chain_bounds(const int* keys, const int64_t* dM, int64_t maxM, int* chainOff,
-                               int* chainEnd, const int* allDone, hipStream_t st);
+                               int* chainEnd, int* longList, int longShift, const int* allDone,
+                               hipStream_t st);
 hipError_t launch_evolve_finalize(int numSegs, int* segDone, int* segMoved, int* segIter, int maxIter,
                                   int* allDone, int* totalIters, hipStream_t st);
 template <typename T>

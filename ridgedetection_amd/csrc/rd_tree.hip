@@ -526,7 +526,7 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cEnd.get(),
                            (int64_t)totalChildren);
         if (ownTotal > 0)
-            check(launch_chain_bounds(sortedKeys.get(), dM.get(), ownTotal, cOff.get(), cEnd.get(), zeroFlag.get(), st_),
+            check(launch_chain_bounds(sortedKeys.get(), dM.get(), ownTotal, cOff.get(), cEnd.get(), nullptr, 0, zeroFlag.get(), st_),
                   "runs");
         std::vector<int> hOff(totalChildren), hEnd(totalChildren);
         check(hipMemcpyAsync(hOff.data(), cOff.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_), "runs");
@@ -579,7 +579,7 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
                                    (int64_t)totalChildren);
                 hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cEnd.get(),
                                    (int64_t)totalChildren);
-                check(launch_chain_bounds(sk.get(), pm.get(), pairs, cOff.get(), cEnd.get(), zeroFlag.get(), st_),
+                check(launch_chain_bounds(sk.get(), pm.get(), pairs, cOff.get(), cEnd.get(), nullptr, 0, zeroFlag.get(), st_),
                       "runs");
                 check(hipMemcpyAsync(nOff.data(), cOff.get(), sizeof(int) * totalChildren, hipMemcpyDeviceToHost, st_),
                       "runs");

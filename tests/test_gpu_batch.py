@@ -18,7 +18,7 @@ def _clouds(dim, count, seed):
         kind = i % 3
         if kind == 0 and dim == 2:
             P = synth.spiral2d(n, seed=seed + i, sigma=0.08)
-        elif kind == 0 or kind == 1:
+        elif (kind == 0 or kind == 1) and dim <= 3:
             P = synth.helix3d(n, seed=seed + i, sigma=0.05)[:, :dim]
         else:
             P = synth.segment_chain(n, dim, seed=seed + i, sigma=0.03)
@@ -37,6 +37,26 @@ def test_batch_matches_single_cloud_oracle(oracle, dim):
         res = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
     for (P, r1, r2), S in zip(clouds, res):
         ref, _ = oracle.ridge_detection(P, r1, r2)
+        np.testing.assert_array_equal(S, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_mixed_dimension_batch(oracle, dtype):
+    """Clouds of dimensions 1..4 in one batch (zero-padded to 4 on upload): every cloud's
+    result equals its own-dimension oracle run bit for bit."""
+    mix = []
+    for d in (2, 3, 4, 1):
+        mix += _clouds(d, 5, 300 + d)
+    g = np.random.Generator(np.random.PCG64(3))
+    order = g.permutation(len(mix))
+    mix = [mix[i] for i in order]
+    print(f"case mixed-dimension batch {np.dtype(dtype).name} clouds={len(mix)}", flush=True)
+    from ridgedetection_amd._lib import Context
+    with Context(4, dtype) as c:
+        res = c.batch([p.astype(dtype) for p, _, _ in mix], [a for _, a, _ in mix], [b for _, _, b in mix])
+    for (P, r1, r2), S in zip(mix, res):
+        assert S.shape[1] == P.shape[1]
+        ref, _ = oracle.ridge_detection(P.astype(dtype), r1, r2)
         np.testing.assert_array_equal(S, ref)
 
 
