@@ -159,6 +159,18 @@ struct WideOf<float> { using type = float4; static constexpr int n = 4; };
 template <>
 struct WideOf<double> { using type = double2; static constexpr int n = 2; };
 
+// sum + v.x + v.y (+ v.z + v.w), in that order
+__device__ __forceinline__ float add_wide(float sum, float4 v) {
+    sum = sum + v.x;
+    sum = sum + v.y;
+    sum = sum + v.z;
+    return sum + v.w;
+}
+__device__ __forceinline__ double add_wide(double sum, double2 v) {
+    sum = sum + v.x;
+    return sum + v.y;
+}
+
 // orders a wave's LDS accesses (the tile is private to the wave)
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -192,20 +204,28 @@ __device__ __forceinline__ T wave_ordered_sum(const T* __restrict__ src, int cnt
         for (int c = 0; c < kLongPer; ++c) tile[c * kWave + lane] = reg[c];
         wave_lds_sync();
         load(t + 1);
+        // groups of kG wide reads: group g + 1 is read while group g is added (the first
+        // pass adds a group of zeros, which leaves the sum unchanged); the scheduling
+        // barriers keep the reads a whole group ahead of their adds
         const W* w = reinterpret_cast<const W*>(tile);
+        constexpr int kG = 8, kNQ = kLongTile / kW;
+        W cur[kG];
 #pragma unroll
-        for (int q = 0; q < kLongTile / kW; ++q) {
-            const W v = w[q];
-            if constexpr (kW == 4) {
-                sum = sum + v.x;
-                sum = sum + v.y;
-                sum = sum + v.z;
-                sum = sum + v.w;
-            } else {
-                sum = sum + v.x;
-                sum = sum + v.y;
-            }
+        for (int g = 0; g < kG; ++g) cur[g] = W{};
+#pragma unroll 2
+        for (int q0 = 0; q0 < kNQ; q0 += kG) {
+            W nxt[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) nxt[g] = w[q0 + g];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) cur[g] = nxt[g];
         }
+#pragma unroll
+        for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
     }
     return sum;
 }
