@@ -207,6 +207,24 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0)):
     return out
 
 
+def batch_cloud(idx, count=512, seed=7):
+    """Cloud `idx` of the batched workload alone (brute-force ridge detection from a host
+    array, eager launches so that a kernel trace sees every iteration)."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context, OPT_GRAPH
+    P, r1, r2 = synth.batched_clouds(count=count, seed=seed)[idx]
+    with Context(P.shape[1], np.float32) as c:
+        c.set_option(OPT_GRAPH, 0)
+        c.set_points(P)
+        t0 = time.perf_counter()
+        S = c.ridge_detection(r1, r2)
+        wall = time.perf_counter() - t0
+        t = c.timers()
+    print(json.dumps({"cloud": idx, "points": len(P), "dim": P.shape[1], "r1": r1, "ridge_points": len(S),
+                      "wall_s": wall, "evolve_iterations": t["evolve_iterations"], "evolve_ms": t["evolve_ms"]}),
+          flush=True)
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (the HIP runtime's copy kernel), repeated 3
     times: the known-size reference the HBM byte counters of the profiled kernels are scaled
@@ -255,15 +273,20 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
     ap.add_argument("--no-tiled", action="store_true", help="skip the tiled measurements")
+    ap.add_argument("--batch-cloud", type=int, default=-1, metavar="IDX",
+                    help="only run cloud IDX of the 512-cloud batch on its own (profiling) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    if args.batch_cloud >= 0:
+        batch_cloud(args.batch_cloud)
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     from ridgedetection_amd import synth
