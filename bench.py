@@ -172,7 +172,7 @@ TILED_NAMES = {0: "local tree, mixed RD", 1: "grouped tree, mixed RD", 2: "local
                4: "grouped tree, local RD", 5: "extended tiles, mixed RD", 6: "extended tiles, local RD"}
 
 
-def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0)):
+def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True):
     """Tiled ridge detection on the helix (1M 3-D points, sigma 0.05, r1 0.2, r2 0.4, 8 tiles
     per dimension, tiles split at 10 000 samples) and on the 6-D Bezier union (2M points, 4
     curves, noise 0.02, 5% outliers, r1 0.08, r2 0.16, 4 tiles per dimension).  For each
@@ -180,7 +180,7 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0)):
     tile's own + neighbour samples read once per iteration, chosen points read, summed and
     updated) over their time, against the 8 TB/s HBM peak."""
     from ridgedetection_amd import synth
-    from ridgedetection_amd._lib import Context
+    from ridgedetection_amd._lib import Context, OPT_GRAPH
     out = {}
     for w in workloads:
         if w == "helix":
@@ -191,6 +191,8 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0)):
             r1, r2, tpd, cap = 0.08, 0.16, [4] * 6, 10000
         for alg in algorithms:
             ctx = Context(P.shape[1], np.float32)
+            if not graph:
+                ctx.set_option(OPT_GRAPH, 0)
             ctx.set_points(P)
             log(f"bench: tiled {w} algorithm {alg}")
             t0 = time.perf_counter()
@@ -273,6 +275,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
     ap.add_argument("--no-tiled", action="store_true", help="skip the tiled measurements")
+    ap.add_argument("--tiled-case", action="append", default=[], metavar="WORKLOAD:ALG",
+                    help="only run these tiled cases (helix|bezier6 : algorithm id), print them and exit")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="tiled cases with eager launches instead of graph replay (kernel traces)")
     ap.add_argument("--batch-cloud", type=int, default=-1, metavar="IDX",
                     help="only run cloud IDX of the 512-cloud batch on its own (profiling) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -283,6 +289,11 @@ def main():
         return
     if args.batch_cloud >= 0:
         batch_cloud(args.batch_cloud)
+        return
+    if args.tiled_case:
+        for case in args.tiled_case:
+            w, alg = case.split(":")
+            print(json.dumps(tiled_runs(workloads=(w,), algorithms=(int(alg),), graph=not args.no_graph)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path; run with --gpus 1",

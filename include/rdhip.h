@@ -117,11 +117,14 @@ enum rdhip_option {
                                       class, times every (samples per lane x threads per block)
                                       scan shape on the data and keeps the fastest (cached per
                                       process); batches keep 4 x 256 */
-    RDHIP_OPT_SEGMENTED = 10       /* 1: a batch whose active segments all have at most 65536
+    RDHIP_OPT_SEGMENTED = 10,      /* 1: a batch whose active segments all have at most 65536
                                       samples and 1024 chosen points iterates as a batch-wide scan
                                       plus one workgroup per segment for the member sort, ordered
                                       sums and shift; 0 (default, faster on the measured tiled and
                                       batched workloads): batch-wide sort and sums */
+    RDHIP_OPT_POSITION_COPY = 11   /* 1 (default): tiled runs copy the samples into segment-position
+                                      order once per tiling, so the evolve and choose passes read them
+                                      without the position -> point index; 0: read through the index */
 };
 
 int rdhip_version(void);

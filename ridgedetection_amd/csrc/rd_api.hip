@@ -400,6 +400,9 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
             case RDHIP_OPT_SEGMENTED:
                 e.options.segmented = value != 0;
                 return RDHIP_OK;
+            case RDHIP_OPT_POSITION_COPY:
+                e.options.positionCopy = value != 0;
+                return RDHIP_OK;
             case RDHIP_OPT_ITERS_PER_SYNC:
                 if (value < 1 || value > 1024) return fail(RDHIP_ERR_INVALID, "iterations per sync must be in [1, 1024]");
                 e.options.itersPerSync = (int)value;

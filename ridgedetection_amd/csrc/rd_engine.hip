@@ -38,6 +38,16 @@ __global__ void row_to_soa_kernel(const T* __restrict__ src, int64_t n, int dim,
         for (int d = 0; d < dim; ++d) dst[d * pStride + i] = src[i * stride + d];
 }
 
+// out[d][i] = P[d][pIndex[i]] for the n positions
+template <typename T>
+__global__ void gather_positions_kernel(const T* __restrict__ P, int64_t pStride, const int* __restrict__ pIndex,
+                                        int64_t n, int dim, T* __restrict__ out, int64_t outStride) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t src = pIndex[i];
+        for (int d = 0; d < dim; ++d) out[d * outStride + i] = P[d * pStride + src];
+    }
+}
+
 // gathers the live chosen points of every segment into a dense output
 template <typename T>
 __global__ void gather_chosen_kernel(const T* __restrict__ S, int64_t sStride, const SegDesc* __restrict__ segs,
@@ -95,6 +105,7 @@ template <typename T>
 void Engine<T>::setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice) {
     if (n < 0) throw std::invalid_argument("negative point count");
     ++version_;
+    psegValid_ = false;
     nPoints_ = n;
     pStride_ = round_up(std::max<int64_t>(n, 1), 64);
     P_.reserve((size_t)pStride_ * dim_);
@@ -138,6 +149,7 @@ void Engine<T>::setSegments(const std::vector<SegDesc>& segs, const std::vector<
                             const std::vector<T>& r1, const std::vector<T>& r2, bool indexed) {
     const int ns = (int)segs.size();
     useIndex_ = indexed;
+    psegValid_ = false;
     if (capacities.size() != segs.size() || r1.size() != segs.size() || r2.size() != segs.size())
         throw std::invalid_argument("segment arrays differ in length");
     ++version_;
@@ -284,6 +296,20 @@ int Engine<T>::persistentBlocks() {
 }
 
 template <typename T>
+void Engine<T>::preparePositions() {
+    if (!useIndex_ || psegValid_ || !options.positionCopy) return;
+    int64_t n = 0;
+    for (auto& s : segs_) n = std::max<int64_t>(n, s.pOff + s.nP);
+    if (n <= 0) return;
+    psegStride_ = round_up(n, 64);
+    Pseg_.reserve((size_t)psegStride_ * dim_);
+    hipLaunchKernelGGL((gather_positions_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st_, P_.get(), pStride_,
+                       pIndexBuf_.get(), n, dim_, Pseg_.get(), psegStride_);
+    check(hipGetLastError(), "gather_positions");
+    psegValid_ = true;
+}
+
+template <typename T>
 void Engine<T>::ensureEvolveScratch() {
     int64_t totalP = 0;
     for (auto& s : segs_) totalP = std::max<int64_t>(totalP, s.pOff + s.nP);
@@ -387,6 +413,8 @@ template <typename T>
 void Engine<T>::choose() {
     const int ns = (int)segs_.size();
     if (ns == 0) return;
+    preparePositions();
+    const EvolveSource es = evolveSource();
     ++version_;
     chosenFromPoints_ = true;
     auto t0 = std::chrono::steady_clock::now();
@@ -404,7 +432,7 @@ void Engine<T>::choose() {
     check(hipMemcpyAsync(dLarge.get(), large.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "large");
     int zero = 0;
     check(hipMemcpyAsync(flags_.get() + 2, &zero, sizeof(int), hipMemcpyHostToDevice, st_), "overflow");
-    ChooseArgs<T> a{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), ns, dSegK_.get(), dSegR1Sqr_.get(),
+    ChooseArgs<T> a{es.P, es.stride, es.index, S_.get(), sStride_, dSegs_.get(), ns, dSegK_.get(), dSegR1Sqr_.get(),
                     covered_.get(), chosenIdx_.get(), flags_.get() + 2, dLarge.get()};
     // small segments: one block each, whole segment
     check(launch_choose_block<T>(a, dim_, -1, 0, st_), "choose_block");
@@ -444,6 +472,7 @@ void Engine<T>::choose() {
 // ------------------------------------------------------------------------------------
 template <typename T>
 void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
+    const EvolveSource es = evolveSource();
     const bool ev = options.profile && slot >= 0;
     if (ev) {
         while ((int)events_.size() < 3 * (slot + 1)) {
@@ -455,12 +484,12 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
     }
     if (segmentedNow_ && !log && !record) {
         // batch-wide scan into labels_, then one workgroup per segment sorts and sums
-        ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(),
+        ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(),
                        dSegK_.get(), dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
                        nullptr, nullptr, nullptr};
         check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
         if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
-        LocalArgs<T> la{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), dSegK_.get(),
+        LocalArgs<T> la{es.P, es.stride, es.index, S_.get(), sStride_, dSegs_.get(), dSegK_.get(),
                         dSegR1Sqr_.get(), nullptr, (int)segs_.size(), segKStride_, segChunk_, 0, 0, nullptr, nullptr,
                         nullptr, labels_.get(), dSegDone_.get(), dSegMoved_.get(), flags_.get()};
         check(launch_segment_sums<T>(la, dim_, (int)segs_.size(), st_), "segment_sums");
@@ -468,7 +497,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
         return;
     }
     int* labels = log ? log->labels : labels_.get();
-    ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+    ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                    dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
                    compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
                    compact() ? flags_.get() + 4 : nullptr};
@@ -485,7 +514,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
     const int64_t kb = std::max<int64_t>(keyBound_, 1);
     // members sorted by (label, position); the last pass writes their coordinates in
     // chain order straight into sortedP_
-    const SortGather<T> sg{P_.get(), pStride_, pIndexPtr(), sortedP_.get(), totalP_, dim_};
+    const SortGather<T> sg{es.P, es.stride, es.index, sortedP_.get(), totalP_, dim_};
     check(stable_sort_by_key<T>(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr, dN_.get(),
                                 totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
                                 sortWs_.get(), st_, flags_.get(), sg),
@@ -496,7 +525,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
     check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
                               longShift, flags_.get(), st_),
           "chain_bounds");
-    ShiftArgs<T> ha{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
+    ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
@@ -533,6 +562,7 @@ inline int ilog2(int64_t v) {
 
 template <typename T>
 void Engine<T>::autotuneScan() {
+    const EvolveSource es = evolveSource();
     // Batches of segments keep the default shape: their iterations are dominated by the
     // per-iteration passes, and the item size also sets the granularity of the active list.
     if (!options.autotune || segs_.size() != 1) {
@@ -562,7 +592,7 @@ void Engine<T>::autotuneScan() {
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
-        ScanArgs<T> sa{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+        ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                        dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
                        nullptr, nullptr, nullptr};
         // three rounds over all candidates (so that no shape always runs first), each
@@ -709,6 +739,7 @@ bool Engine<T>::chooseSegmented(const std::vector<int>& done) {
 template <typename T>
 int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
     const int ns = (int)segs_.size();
+    const EvolveSource es = evolveSource();
     std::vector<int> K = countsHost();
     // Taking every active segment off the batch-wide iterations removes their fixed
     // per-iteration cost entirely; taking only some keeps it, so then only the smallest go.
@@ -746,7 +777,7 @@ int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
           "local list");
     int zero2[2] = {0, 0};
     check(hipMemcpyAsync(flags_.get() + 5, zero2, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "local flags");
-    LocalArgs<T> a{P_.get(), pStride_, pIndexPtr(), S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegR1Sqr_.get(),
+    LocalArgs<T> a{es.P, es.stride, es.index, S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegR1Sqr_.get(),
                    dLocalList_.get(), (int)list.size(), (int)round_up(std::max(maxK, 1), kChosenTile),
                    std::min(std::max(maxP, 1), kLocalChunk), maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0,
                    (int)kEvolveGuard, dSegIter_.get(), flags_.get() + 5, flags_.get() + 6,
@@ -764,6 +795,7 @@ template <typename T>
 long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) {
     const int ns = (int)segs_.size();
     if (ns == 0) return 0;
+    preparePositions();
     ++version_;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<int> done(ns, 0), zeros(ns, 0);

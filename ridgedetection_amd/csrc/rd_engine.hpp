@@ -90,6 +90,7 @@ struct RunOptions {
     int64_t localMaxWork = 1 << 22; // ... and samples x chosen points at most this
     bool segmented = false;         // batches whose active segments all have <= segmentedMaxP
     int segmentedMaxP = 65536;      // samples: batch-wide scan + per-segment sort and sums
+    bool positionCopy = true;       // indexed segments: evolve reads a position-ordered copy
 };
 
 template <typename T>
@@ -193,6 +194,22 @@ private:
     DevBuf<int> pIndexBuf_;
     bool useIndex_ = false;
     const int* pIndexPtr() const { return useIndex_ ? pIndexBuf_.get() : nullptr; }
+    // samples as the evolve passes read them: the points through the position index, or
+    // (indexed segments, positionCopy) a copy in position order, written once per points /
+    // segments setting, read without the index
+    struct EvolveSource {
+        const T* P;
+        int64_t stride;
+        const int* index;
+    };
+    DevBuf<T> Pseg_;
+    int64_t psegStride_ = 0;
+    bool psegValid_ = false;
+    void preparePositions();
+    EvolveSource evolveSource() const {
+        if (useIndex_ && psegValid_ && options.positionCopy) return {Pseg_.get(), psegStride_, nullptr};
+        return {P_.get(), pStride_, pIndexPtr()};
+    }
     // segments
     std::vector<SegDesc> segs_;
     std::vector<int> groupBegin_, groupEnd_;
