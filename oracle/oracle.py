@@ -203,13 +203,15 @@ class Oracle:
                     "evolve_vjp")
         return gp, g
 
-    def decimate(self, S, r2):
+    def decimate(self, S, r2, nocopy=False):
+        """decimate (or, nocopy=True, decimateNoCopy over one list); returns the kept points
+        in order and their indices."""
         S = np.array(S, copy=True, order="C")
         K, d = S.shape
         kept = np.empty(max(K, 1), dtype=np.int64)
         if self.prefix == "oracle_":
             k = self._fn("decimate", S.dtype)(_p(S), C.c_long(K), C.c_int(d), self._scalar(S.dtype, r2),
-                                               C.c_int(0), _p(kept))
+                                               C.c_int(1 if nocopy else 0), _p(kept))
         else:
             k = self._fn("decimate", S.dtype)(_p(S), C.c_long(K), C.c_int(d), self._scalar(S.dtype, r2),
                                                _p(kept))

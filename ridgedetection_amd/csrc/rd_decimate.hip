@@ -14,10 +14,21 @@
 //   2. count    -- c1(e) = #{j : d2 <= r2^2}, c2(e) = #{j : d2 <= 4 r2^2} over the group
 //                  (uniform inner loop: the partner coordinates are scalar loads);
 //   3. fill     -- CSR list of the 2 r2 neighbours of every element, with a "within r2" bit;
-//   4. scan     -- one wave per group walks the elements in order, 64 at a time: the
-//                  window's counters and CSR rows are staged in LDS, decisions are uniform,
-//                  and a removal decrements its neighbours' counters (LDS for the window,
-//                  device-scope atomics beyond it);
+//   4. scan     -- groups below kDecRoundsMin elements: one wave per group walks the
+//                  elements in order, 64 at a time: the window's counters and CSR rows are
+//                  staged in LDS, decisions are uniform, and a removal decrements its
+//                  neighbours' counters (LDS for the window, device-scope atomics beyond it);
+//      rounds   -- larger groups, one pass at a time: element e's test depends only on
+//                  which of its earlier (lower-index) neighbours the pass removed, since
+//                  its later neighbours are still live when the walk reaches it.  So in
+//                  every round each undecided element resumes the scan of its earlier
+//                  neighbours (rows are in increasing index order) and decides once all of
+//                  them have decided; the earliest undecided element always can, so every
+//                  round makes progress, and random-like orders need few rounds.  The
+//                  group-wide count rule (stop below 3 / keep below 3) can only act when the
+//                  pass would leave fewer than 3 points: such a pass is discarded and the
+//                  group resumes in the serial walk from the pass-start state.  Otherwise
+//                  the removals apply and the counts are recounted from the CSR rows;
 //   5. compact  -- stable in-place compaction of every segment's chosen points.
 #include "rd_decimate.hpp"
 #include "rd_sort.hpp"
@@ -61,10 +72,44 @@ __global__ __launch_bounds__(256) void dec_elems_kernel(DecArgs a) {
     }
 }
 
+// Bounding box of every chunk of kDecChunk consecutive elements: lo at [c][0, DIM), hi at
+// [c][DIM, 2 DIM).  One wave per chunk.
+constexpr int kDecChunk = 64;
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* __restrict__ S, int64_t ss,
+                                                            T* __restrict__ boxes, int64_t maxChunks) {
+    const int64_t c = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+    if (c >= maxChunks) return;
+    const int64_t total = *a.totalElems;
+    const int64_t e = c * kDecChunk + lane_id();
+    T lo[DIM], hi[DIM];
+    const int se = e < total ? a.elemSlot[e] : 0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const T x = e < total ? S[d * ss + se] : T(0);
+        lo[d] = e < total ? x : FpTraits<T>::big;
+        hi[d] = e < total ? x : -FpTraits<T>::big;
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            lo[d] = min(lo[d], __shfl_xor(lo[d], m));
+            hi[d] = max(hi[d], __shfl_xor(hi[d], m));
+        }
+    if (lane_id() == 0)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            boxes[c * 2 * DIM + d] = lo[d];
+            boxes[c * 2 * DIM + DIM + d] = hi[d];
+        }
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __restrict__ S, int64_t ss,
                                                         const T* __restrict__ groupRSqr, int fill,
-                                                        int64_t maxElems) {
+                                                        int64_t maxElems, const T* __restrict__ boxes) {
     const int64_t total = *a.totalElems;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) {
@@ -81,17 +126,29 @@ __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __re
     for (int d = 0; d < DIM; ++d) p[d] = S[d * ss + se];
     int c1 = 0, c2 = 0;
     int w = fill ? a.rowOff[e] : 0;
-    for (int j = o; j < o + n; ++j) {
-        const int sj = a.elemSlot[j];
-        T q[DIM];
+    // Chunks whose box lies farther than 2 r2 hold no neighbour: the squared distance to a
+    // box bounds every member's from below; the margin covers the rounding of both sums.
+    const T skipAbove = r4 * (T(1) + T(16 * (DIM + 2)) * FpTraits<T>::unit_roundoff) + FpTraits<T>::tiny;
+    for (int c = o / kDecChunk, cEnd = n > 0 ? (o + n - 1) / kDecChunk : c - 1; c <= cEnd; ++c) {
+        T bd = T(0);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) q[d] = S[d * ss + sj];
-        const T dd = exact_sqdist<T, DIM>(p, q);
-        const bool in1 = dd <= rSqr;
-        const bool in2 = dd <= r4;
-        c1 += in1;
-        c2 += in2;
-        if (fill && in2 && j != e) a.nbr[w++] = (j << 1) | (in1 ? 1 : 0);
+        for (int d = 0; d < DIM; ++d) {
+            const T t = max(max(boxes[c * 2 * DIM + d] - p[d], p[d] - boxes[c * 2 * DIM + DIM + d]), T(0));
+            bd = bd + t * t;
+        }
+        if (bd > skipAbove) continue;
+        for (int j = max(o, c * kDecChunk), j1 = min(o + n, (c + 1) * kDecChunk); j < j1; ++j) {
+            const int sj = a.elemSlot[j];
+            T q[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) q[d] = S[d * ss + sj];
+            const T dd = exact_sqdist<T, DIM>(p, q);
+            const bool in1 = dd <= rSqr;
+            const bool in2 = dd <= r4;
+            c1 += in1;
+            c2 += in2;
+            if (fill && in2 && j != e) a.nbr[w++] = (j << 1) | (in1 ? 1 : 0);
+        }
     }
     if (!fill) {
         a.c1[e] = c1;
@@ -106,11 +163,15 @@ __global__ __launch_bounds__(64) void dec_scan_kernel(DecArgs a) {
     __shared__ int wc1[kWave], wc2[kWave], wal[kWave];
     __shared__ int rows[kRowCap];
     const int g = blockIdx.x;
+    const int mode = a.groupMode[g];
+    if (mode != kDecSerial && mode != kDecResume) return;
     const int o = a.groupOff[g], n = a.groupCnt[g];
     const int lane = lane_id();
-    for (int e = o + lane; e < o + n; e += kWave) st_rel(&a.alive[e], 1);
+    if (mode == kDecSerial)
+        for (int e = o + lane; e < o + n; e += kWave) st_rel(&a.alive[e], 1);
     __threadfence();
-    int cnt = n;
+    // a resumed group continues from the start of its current pass
+    int cnt = mode == kDecSerial ? n : a.groupAlive[g];
     int left = -1;
     bool stop = false;
     while (left != cnt && cnt > 3 && !stop) {
@@ -168,7 +229,168 @@ __global__ __launch_bounds__(64) void dec_scan_kernel(DecArgs a) {
             __syncthreads();
         }
     }
-    if (lane == 0) a.groupLeft[g] = cnt;
+    if (lane == 0) {
+        a.groupLeft[g] = cnt;
+        a.groupMode[g] = kDecDone;
+    }
+}
+
+// ---- dependency rounds --------------------------------------------------------------
+__global__ void dec_modes_kernel(DecArgs a) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.numGroups) return;
+    const int n = a.groupCnt[g];
+    const bool rounds = n >= kDecRoundsMin;
+    a.groupMode[g] = rounds ? kDecRounds : kDecSerial;
+    a.groupAlive[g] = n;
+    a.groupRemoved[g] = 0;
+    a.groupApply[g] = 0;
+    a.groupLeft[g] = n;
+    if (rounds) atomicAdd(&a.parFlags[1], 1);
+}
+
+// every element of a group in rounds mode: live, and undecided for the coming pass
+__global__ void dec_pass_reset_kernel(DecArgs a, int first) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= *a.totalElems) return;
+    if (a.groupMode[a.elemGroup[e]] != kDecRounds) return;
+    if (first) a.alive[e] = 1;
+    a.state[e] = 0;
+    a.prog[e] = a.rowOff[e];
+    a.rc1[e] = 0;
+    a.rc2[e] = 0;
+}
+
+// One round: an undecided element resumes the scan of its earlier neighbours.  Those
+// still undecided bound its counts: with r1 / r2 removals known and u1 / u2 undecided (r2
+// and 2 r2 neighbourhoods), x1 lies in [c1 - r1 - u1, c1 - r1] and x2 in [c2 - r2 - u2,
+// c2 - r2], so the test is already settled when it holds (or fails) at both ends -- in a
+// dense neighbourhood every point with at least three later neighbours within r2 is
+// removed whatever the earlier ones do.  An unsettled element looks again, at most
+// kDecRetries times (neighbours decided meanwhile by other waves are then seen), before
+// leaving it to the next round.  The decided prefix of the row is folded into prog / rc.
+constexpr int kDecRetries = 48;
+
+__device__ __forceinline__ bool dec_removes(int x1, int x2) { return x1 >= 4 || x2 < 3; }
+
+__global__ __launch_bounds__(256) void dec_round_kernel(DecArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= *a.totalElems) return;
+    if (a.groupMode[a.elemGroup[e]] != kDecRounds) return;
+    if (!a.alive[e] || a.state[e] != 0) return;
+    int q0 = a.prog[e];
+    const int qe = a.rowOff[e + 1];
+    int r1 = a.rc1[e], r2 = a.rc2[e];   // removals among the folded prefix [rowOff, q0)
+    const int c1 = a.c1[e], c2 = a.c2[e];
+    for (int attempt = 0;; ++attempt) {
+        int k1 = 0, k2 = 0, u1 = 0, u2 = 0;   // removals / undecided past the prefix
+        bool prefix = true;
+        for (int q = q0; q < qe; ++q) {
+            const int v = a.nbr[q];
+            const int j = v >> 1;
+            if (j > e) break;                 // rows are ascending: no earlier neighbours left
+            int sj = 1;
+            if (a.alive[j]) sj = ld_acq(&a.state[j]);
+            if (sj == 0) {
+                prefix = false;
+                ++u2;
+                u1 += v & 1;
+            } else {
+                const int rm = sj == 2 ? 1 : 0;
+                if (prefix) {
+                    q0 = q + 1;
+                    r2 += rm;
+                    r1 += rm & v;
+                } else {
+                    k2 += rm;
+                    k1 += rm & v;
+                }
+            }
+        }
+        const int hi1 = c1 - r1 - k1, hi2 = c2 - r2 - k2;   // no further removals
+        const int lo1 = hi1 - u1, lo2 = hi2 - u2;            // every undecided one removed
+        // the test rises with x1 and falls with x2: it holds over the whole box iff it holds
+        // at (lo1, hi2), and fails over it iff it fails at (hi1, lo2)
+        const bool surelyRemoved = dec_removes(lo1, hi2);
+        const bool surelyKept = !dec_removes(hi1, lo2);
+        if (surelyRemoved || surelyKept) {
+            st_rel(&a.state[e], surelyRemoved ? 2 : 1);
+            return;
+        }
+        if (attempt + 1 == kDecRetries) {
+            a.prog[e] = q0;
+            a.rc1[e] = r1;
+            a.rc2[e] = r2;
+            st_rel(&a.parFlags[0], 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+__global__ __launch_bounds__(256) void dec_pass_sum_kernel(DecArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int g = -1;
+    bool rm = false;
+    if (e < *a.totalElems) {
+        g = a.elemGroup[e];
+        rm = a.groupMode[g] == kDecRounds && a.alive[e] && a.state[e] == 2;
+    }
+    // one atomic per wave when its elements share a group (the common case)
+    const int g0 = __shfl(g, 0);
+    const bool uniform = __all(g == g0);
+    if (uniform) {
+        const int c = __popcll(__ballot(rm));
+        if (lane_id() == 0 && c > 0) atomicAdd(&a.groupRemoved[g0], c);
+    } else if (rm) {
+        atomicAdd(&a.groupRemoved[g], 1);
+    }
+}
+
+__global__ void dec_pass_decide_kernel(DecArgs a) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= a.numGroups) return;
+    a.groupApply[g] = 0;
+    if (a.groupMode[g] != kDecRounds) return;
+    const int n = a.groupAlive[g], R = a.groupRemoved[g];
+    a.groupRemoved[g] = 0;
+    if (n - R < 3) {                   // the count rule would act: redo this pass serially
+        a.groupMode[g] = kDecResume;
+        atomicAdd(&a.parFlags[2], 1);
+        return;
+    }
+    a.groupApply[g] = 1;
+    a.groupAlive[g] = n - R;
+    a.groupLeft[g] = n - R;
+    if (R > 0 && n - R > 3) {
+        atomicAdd(&a.parFlags[1], 1);
+    } else {
+        a.groupMode[g] = kDecDone;
+    }
+}
+
+__global__ void dec_pass_apply_kernel(DecArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= *a.totalElems) return;
+    if (!a.groupApply[a.elemGroup[e]]) return;
+    if (a.alive[e] && a.state[e] == 2) a.alive[e] = 0;
+}
+
+// counts for the next pass over the live elements (itself included)
+__global__ void dec_recount_kernel(DecArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= *a.totalElems) return;
+    if (a.groupMode[a.elemGroup[e]] != kDecRounds || !a.alive[e]) return;
+    int c1 = 1, c2 = 1;
+    for (int q = a.rowOff[e], qe = a.rowOff[e + 1]; q < qe; ++q) {
+        const int v = a.nbr[q];
+        if (a.alive[v >> 1]) {
+            ++c2;
+            c1 += v & 1;
+        }
+    }
+    a.c1[e] = c1;
+    a.c2[e] = c2;
 }
 
 template <typename T, int DIM>
@@ -220,6 +442,66 @@ __global__ __launch_bounds__(256) void dec_compact_kernel(DecArgs a, T* __restri
     if (threadIdx.x == 0) a.segK[s] = newK;
 }
 
+namespace {
+hipError_t dec_ordered_removal(DecArgs& a, int64_t maxElems, DecWorkspace& ws, hipStream_t st) {
+    hipError_t err;
+    const int64_t need = 5 * std::max<int64_t>(maxElems, 1) + 4 * (int64_t)a.numGroups + 4;
+    if (need > ws.parCap) {
+        if (ws.par) (void)hipFree(ws.par);
+        ws.par = nullptr;
+        ws.parCap = need + need / 4;
+        if ((err = hipMalloc(&ws.par, sizeof(int) * ws.parCap)) != hipSuccess) return err;
+    }
+    int* p = ws.par;
+    const int64_t me = std::max<int64_t>(maxElems, 1);
+    a.state = p; p += me;
+    a.prog = p; p += me;
+    a.rc1 = p; p += me;
+    a.rc2 = p; p += me;
+    p += me;
+    a.groupMode = p; p += a.numGroups;
+    a.groupAlive = p; p += a.numGroups;
+    a.groupRemoved = p; p += a.numGroups;
+    a.groupApply = p; p += a.numGroups;
+    a.parFlags = p;
+    const unsigned gb = (unsigned)((a.numGroups + 255) / 256);
+    const unsigned eb = (unsigned)((me + 255) / 256);
+    int flags[3] = {0, 0, 0};
+    if ((err = hipMemsetAsync(a.parFlags, 0, sizeof(int) * 3, st)) != hipSuccess) return err;
+    hipLaunchKernelGGL(dec_modes_kernel, dim3(gb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(dec_scan_kernel, dim3(a.numGroups), dim3(kWave), 0, st, a);
+    if ((err = hipMemcpyAsync(flags, a.parFlags, sizeof(int) * 3, hipMemcpyDeviceToHost, st)) != hipSuccess) return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+    bool first = true;
+    while (flags[1] > 0) {
+        if ((err = hipMemsetAsync(a.parFlags, 0, sizeof(int) * 3, st)) != hipSuccess) return err;
+        hipLaunchKernelGGL(dec_pass_reset_kernel, dim3(eb), dim3(256), 0, st, a, first ? 1 : 0);
+        first = false;
+        // rounds until no element is left undecided; every round decides at least the
+        // earliest undecided element of each group
+        for (int64_t done = 0; done <= maxElems; done += 8) {
+            for (int r = 0; r < 7; ++r) hipLaunchKernelGGL(dec_round_kernel, dim3(eb), dim3(256), 0, st, a);
+            if ((err = hipMemsetAsync(a.parFlags, 0, sizeof(int), st)) != hipSuccess) return err;
+            hipLaunchKernelGGL(dec_round_kernel, dim3(eb), dim3(256), 0, st, a);
+            if ((err = hipMemcpyAsync(flags, a.parFlags, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
+                return err;
+            if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+            if (!flags[0]) break;
+        }
+        if (flags[0]) return hipErrorUnknown;   // unreachable: rounds always make progress
+        hipLaunchKernelGGL(dec_pass_sum_kernel, dim3(eb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(dec_pass_decide_kernel, dim3(gb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(dec_pass_apply_kernel, dim3(eb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(dec_recount_kernel, dim3(eb), dim3(256), 0, st, a);
+        if ((err = hipMemcpyAsync(flags, a.parFlags, sizeof(int) * 3, hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return err;
+        if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+        if (flags[2] > 0) hipLaunchKernelGGL(dec_scan_kernel, dim3(a.numGroups), dim3(kWave), 0, st, a);
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
 template <typename T>
 hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* groupRSqr, int* chosenIdx,
                            unsigned char* keepOut, int64_t maxElems, DecWorkspace& ws, hipStream_t st) {
@@ -232,12 +514,23 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
         return err;
     hipLaunchKernelGGL(dec_groups_kernel, dim3((a.numGroups + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(dec_elems_kernel, dim3(a.numSegs), dim3(256), 0, st, a);
-    // 2. counts
+    // 2. chunk boxes, counts
+    const int64_t maxChunks = std::max<int64_t>((maxElems + kDecChunk - 1) / kDecChunk, 1);
+    const int64_t boxBytes = maxChunks * 2 * dim * (int64_t)sizeof(T);
+    if (boxBytes > ws.boxCap) {
+        if (ws.boxes) (void)hipFree(ws.boxes);
+        ws.boxes = nullptr;
+        ws.boxCap = boxBytes + boxBytes / 4;
+        if ((err = hipMalloc(&ws.boxes, ws.boxCap)) != hipSuccess) return err;
+    }
+    T* boxes = static_cast<T*>(ws.boxes);
     const unsigned blocks = (unsigned)((maxElems + 1 + 255) / 256);
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
+        hipLaunchKernelGGL((dec_chunk_box_kernel<T, DIM>), dim3((unsigned)((maxChunks + 3) / 4)), dim3(256), 0, st, a,
+                           (const T*)S, sStride, boxes, maxChunks);
         hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, S, sStride, groupRSqr,
-                           0, maxElems);
+                           0, maxElems, (const T*)boxes);
         ok = true;
     });
     if (!ok) return hipErrorInvalidValue;
@@ -256,10 +549,10 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     a.nbr = ws.nbr;
     RD_DISPATCH_DIM(dim, {
         hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, S, sStride, groupRSqr,
-                           1, maxElems);
+                           1, maxElems, (const T*)boxes);
     });
-    // 4. ordered removal, one wave per group
-    hipLaunchKernelGGL(dec_scan_kernel, dim3(a.numGroups), dim3(kWave), 0, st, a);
+    // 4. ordered removal: small groups one wave each; large groups in dependency rounds
+    if ((err = dec_ordered_removal(a, maxElems, ws, st)) != hipSuccess) return err;
     // 5. compaction
     RD_DISPATCH_DIM(dim, {
         hipLaunchKernelGGL((dec_compact_kernel<T, DIM>), dim3(a.numSegs), dim3(256), 0, st, a, S, sStride,

@@ -28,13 +28,31 @@ struct DecArgs {
     int* nbr;
     int* alive;
     const int* segSkip;        // optional, per segment: leave this segment untouched
+    // groups of at least kDecRoundsMin elements decide in dependency rounds (set by
+    // decimate_groups from its workspace)
+    int* groupMode;            // kDecSerial / kDecRounds / kDecResume / kDecDone
+    int* groupAlive;           // live elements at the start of the current pass
+    int* groupRemoved;
+    int* groupApply;
+    int* state;                // per element in the current pass: 0 undecided, 1 kept, 2 removed
+    int* prog;                 // next CSR entry an undecided element examines
+    int* rc1;                  // removed earlier neighbours within r2 / 2 r2 found so far
+    int* rc2;
+    int* parFlags;             // [0] an element is still undecided, [1] groups in rounds, [2] groups to resume
 };
+
+constexpr int kDecRoundsMin = 2048;
+enum : int { kDecSerial = 0, kDecRounds = 1, kDecResume = 2, kDecDone = 3 };
 
 struct DecWorkspace {
     int* partials = nullptr;   // scan partials for max(numSegs, maxElems + 1)
     int64_t* nbrTotal = nullptr;
     int* nbr = nullptr;
     int64_t nbrCap = 0;
+    int* par = nullptr;        // dependency-round state (DecArgs::groupMode ... parFlags)
+    int64_t parCap = 0;
+    void* boxes = nullptr;     // bounding boxes of element chunks (value type of the call)
+    int64_t boxCap = 0;        // bytes
 };
 
 // Decimates every group; chosenIdx (optional) is compacted alongside the coordinates and

@@ -98,6 +98,8 @@ template <typename T>
 Engine<T>::~Engine() {
     for (auto ev : events_) (void)hipEventDestroy(ev);
     if (decWs_.nbr) (void)hipFree(decWs_.nbr);
+    if (decWs_.par) (void)hipFree(decWs_.par);
+    if (decWs_.boxes) (void)hipFree(decWs_.boxes);
     for (auto* b : chooseItems_) delete b;
 }
 
