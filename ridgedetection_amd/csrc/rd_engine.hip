@@ -297,6 +297,24 @@ int Engine<T>::persistentBlocks() {
     return persistentBlocks_;
 }
 
+// Slots of the segments still evolving, when far fewer than the slot range (tiles keep one
+// slot per own sample): the sum rows of this evolve call visit only those.
+template <typename T>
+void Engine<T>::prepareRows(const std::vector<int>& done) {
+    numRowSlots_ = 0;
+    const std::vector<int> K = countsHost();
+    std::vector<int> rows;
+    for (size_t s = 0; s < segs_.size(); ++s)
+        if (!done[s])
+            for (int k = 0; k < K[s]; ++k) rows.push_back((int)(segs_[s].sOff + k));
+    if (rows.empty() || 4 * (int64_t)rows.size() > std::max<int64_t>(keyBound_, 1)) return;
+    rowSlots_.reserve(rows.size());
+    check(hipMemcpyAsync(rowSlots_.get(), rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, st_),
+          "row slots");
+    numRowSlots_ = (int64_t)rows.size();
+    rowSlotsVersion_ = version_;
+}
+
 template <typename T>
 void Engine<T>::preparePositions() {
     if (!useIndex_ || psegValid_ || !options.positionCopy) return;
@@ -532,7 +550,12 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
                     log ? log->moved : (record ? moved_.get() : nullptr),
-                    log ? log->movedStride : totalSlots_, longList_.get(), longShift};
+                    log ? log->movedStride : totalSlots_, longList_.get(), longShift, nullptr};
+    if (rowSlotsVersion_ == version_ && numRowSlots_ > 0) {
+        // rows over the live slots of the evolve call only
+        ha.totalSlots = numRowSlots_;
+        ha.rowSlots = rowSlots_.get();
+    }
     check(launch_chain_sum_shift<T>(ha, dim_, totalP, false, st_), "chain_sum_shift");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
 }
@@ -834,6 +857,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     } else if (compact()) {
         rebuildActive(done);
     }
+    prepareRows(done);
     const int capIter = maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0;
     long long iters = 0;
     const int batch = std::max(1, options.itersPerSync);

@@ -206,6 +206,10 @@ private:
     int64_t psegStride_ = 0;
     bool psegValid_ = false;
     void preparePositions();
+    DevBuf<int> rowSlots_;
+    int64_t numRowSlots_ = 0;
+    uint64_t rowSlotsVersion_ = ~uint64_t(0);
+    void prepareRows(const std::vector<int>& done);
     EvolveSource evolveSource() const {
         if (useIndex_ && psegValid_ && options.positionCopy) return {Pseg_.get(), psegStride_, nullptr};
         return {P_.get(), pStride_, pIndexPtr()};

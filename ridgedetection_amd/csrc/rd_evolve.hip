@@ -111,7 +111,8 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
     const int t = threadIdx.x & (kRow - 1);
     const int64_t row = blk * (256 / kRow) + threadIdx.x / kRow;
     const int d = (int)(row / max<int64_t>(a.totalSlots, 1));
-    const int64_t slot = row - (int64_t)d * a.totalSlots;
+    const int64_t rs = row - (int64_t)d * a.totalSlots;
+    const int64_t slot = (a.rowSlots && d < DIM) ? (int64_t)a.rowSlots[rs] : rs;
     int seg = -1, off = 0, cnt = 0;
     if (d < DIM) {
         seg = a.slotSeg[slot];

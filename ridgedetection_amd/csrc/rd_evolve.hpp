@@ -61,6 +61,9 @@ struct ShiftArgs {
     // longList[m] is the chain whose first such position is m L, else -1 (chain_bounds)
     const int* longList;
     int longShift;
+    // optional: the slots the rows visit (row r: coordinate r / totalSlots, slot
+    // rowSlots[r % totalSlots]); null: every slot below totalSlots
+    const int* rowSlots;
 };
 
 // L of the wave-summed chains for up to maxM sorted members: the smallest power of two
