@@ -6,7 +6,13 @@
 //   scatter -- per-block stable ranking: every wave ranks its 64 keys with a ballot
 //              match (one ballot per digit bit), carries per-digit counters across its
 //              chunks in LDS, and the four waves of a block are stitched together by a
-//              per-digit prefix, so equal keys keep their input order.
+//              per-digit prefix, so equal keys keep their input order.  For digits of at
+//              most kStageBits bits (runs of at least 8 pairs per digit and block on
+//              average) the block then stages its pairs in LDS in block-local sorted order
+//              and writes them out in that order: consecutive lanes write consecutive
+//              addresses inside each digit's run instead of 64 scattered runs per store.
+//              Wider digits leave runs too short to gain, and the larger LDS footprint
+//              would halve the blocks per CU, so they write straight from the ranks.
 // Negative keys are dropped by the first pass, which also writes the count of kept
 // pairs.  Used to list the members of every chosen point in point order (evolve) and to
 // bin points by tile (tree build).
@@ -149,12 +155,21 @@ struct GatherSink {
     }
 };
 
-template <typename Sink>
+constexpr int kStageBits = 9;
+
+template <bool kStage, typename Sink>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
     const int* __restrict__ keysIn, const int* __restrict__ valsIn, const int64_t* __restrict__ nPtr,
     int shift, int bits, int numBlocks, const int* __restrict__ ghistScanned,
     int* __restrict__ keysOut, Sink sink, const int* __restrict__ skip) {
-    __shared__ int cnt[kWavesPerBlock][1 << kMaxDigitBits];
+    constexpr int kDigits = 1 << (kStage ? kStageBits : kMaxDigitBits);
+    constexpr int kPer = (kDigits + kThreads - 1) / kThreads;
+    // per-wave digit counters, later reused for the staged (destination, key, value) triples
+    constexpr int kSmem = kStage ? std::max(kWavesPerBlock * kDigits, 3 * kBlockElems) : kWavesPerBlock * kDigits;
+    __shared__ int smem[kSmem];
+    __shared__ int lbase[kStage ? kDigits : 1];
+    __shared__ int wsum[kWavesPerBlock];
+    int (*cnt)[kDigits] = reinterpret_cast<int (*)[kDigits]>(smem);
     if (skip && *skip) return;
     const int nb = 1 << bits;
     const int mask = nb - 1;
@@ -194,30 +209,120 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
         pos[c] = p;
     }
     __syncthreads();
-    for (int v = threadIdx.x; v < nb; v += kThreads) {
-        int run = ghistScanned[(int64_t)v * numBlocks + blockIdx.x];
+    if constexpr (!kStage) {
+        for (int v = threadIdx.x; v < nb; v += kThreads) {
+            int run = ghistScanned[(int64_t)v * numBlocks + blockIdx.x];
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                int t = cnt[w][v];
+                cnt[w][v] = run;
+                run += t;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int k = key[c];
+            if (k >= 0) {
+                const int64_t i = w0 + c * kWave + lane;
+                const int d = cnt[wave][(k >> shift) & mask] + pos[c];
+                keysOut[d] = k;
+                sink.put(d, valsIn ? valsIn[i] : (int)i);
+            }
+        }
+    } else {
+        // block-local digit bases: exclusive scan of the block's per-digit totals
+        int tot[kPer];
+        int tsum = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int v = threadIdx.x * kPer + j;
+            int t = 0;
+            if (v < nb)
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w) t += cnt[w][v];
+            tot[j] = t;
+            tsum += t;
+        }
+        int incl = tsum;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        if (lane == kWave - 1) wsum[wave] = incl;
+        __syncthreads();
+        int excl = incl - tsum;
+        int blockCount = 0;
 #pragma unroll
         for (int w = 0; w < kWavesPerBlock; ++w) {
-            int t = cnt[w][v];
-            cnt[w][v] = run;
-            run += t;
+            excl += w < wave ? wsum[w] : 0;
+            blockCount += wsum[w];
         }
-    }
-    __syncthreads();
 #pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-        int k = key[c];
-        if (k >= 0) {
-            int64_t i = w0 + c * kWave + lane;
-            int dig = (k >> shift) & mask;
-            int dst = cnt[wave][dig] + pos[c];
-            keysOut[dst] = k;
-            sink.put(dst, valsIn ? valsIn[i] : (int)i);
+        for (int j = 0; j < kPer; ++j) {
+            const int v = threadIdx.x * kPer + j;
+            if (v < nb) lbase[v] = excl;
+            excl += tot[j];
+        }
+        // global bases per (wave, digit)
+        for (int v = threadIdx.x; v < nb; v += kThreads) {
+            int run = ghistScanned[(int64_t)v * numBlocks + blockIdx.x];
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                int t = cnt[w][v];
+                cnt[w][v] = run;
+                run += t;
+            }
+        }
+        __syncthreads();
+        int dst[kChunks], rank[kChunks];
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int k = key[c];
+            dst[c] = 0;
+            rank[c] = 0;
+            if (k >= 0) {
+                const int dig = (k >> shift) & mask;
+                dst[c] = cnt[wave][dig] + pos[c];
+                rank[c] = lbase[dig] + (cnt[wave][dig] - cnt[0][dig]) + pos[c];
+            }
+        }
+        __syncthreads();   // the counters are dead: stage the triples in block-local order
+        int* sDst = smem;
+        int* sKey = smem + kBlockElems;
+        int* sVal = smem + 2 * kBlockElems;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int k = key[c];
+            if (k >= 0) {
+                const int64_t i = w0 + c * kWave + lane;
+                sDst[rank[c]] = dst[c];
+                sKey[rank[c]] = k;
+                sVal[rank[c]] = valsIn ? valsIn[i] : (int)i;
+            }
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < blockCount; r += kThreads) {
+            const int d = sDst[r];
+            keysOut[d] = sKey[r];
+            sink.put(d, sVal[r]);
         }
     }
 }
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <typename Sink>
+void launch_scatter(const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks, const int* ghist,
+                    int* oK, const Sink& sink, const int* skip, hipStream_t stream) {
+    if (w <= kStageBits)
+        hipLaunchKernelGGL((radix_scatter_kernel<true, Sink>), dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
+                           inV, n, shift, w, blocks, ghist, oK, sink, skip);
+    else
+        hipLaunchKernelGGL((radix_scatter_kernel<false, Sink>), dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
+                           inV, n, shift, w, blocks, ghist, oK, sink, skip);
+}
 
 }  // namespace
 
@@ -287,8 +392,7 @@ hipError_t sort_passes(const int* keysIn, const int* valsIn, const int64_t* dN, 
         if (p == passes - 1)
             lastScatter(inK, inV, n, shift, w, (int)blocks, ghist, oK, oV);
         else
-            hipLaunchKernelGGL(radix_scatter_kernel<ValueSink>, dim3((unsigned)blocks), dim3(kThreads), 0, stream,
-                               inK, inV, n, shift, w, (int)blocks, ghist, oK, ValueSink{oV}, skip);
+            launch_scatter(inK, inV, n, shift, w, (int)blocks, ghist, oK, ValueSink{oV}, skip, stream);
         inK = oK;
         inV = oV;
         n = dM;
@@ -304,9 +408,7 @@ hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_
     return sort_passes(keysIn, valsIn, dN, maxN, numKeys, keysOut, valsOut, dM, workspace, stream, skip,
                        [&](const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks,
                            const int* ghist, int* oK, int* oV) {
-                           hipLaunchKernelGGL(radix_scatter_kernel<ValueSink>, dim3((unsigned)blocks), dim3(kThreads),
-                                              0, stream, inK, inV, n, shift, w, blocks, ghist, oK, ValueSink{oV},
-                                              skip);
+                           launch_scatter(inK, inV, n, shift, w, blocks, ghist, oK, ValueSink{oV}, skip, stream);
                        });
 }
 
@@ -320,9 +422,8 @@ hipError_t stable_sort_by_key(const int* keysIn, const int* valsIn, const int64_
         [&](const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks, const int* ghist,
             int* oK, int*) {
             RD_DISPATCH_DIM(g.dim, {
-                hipLaunchKernelGGL((radix_scatter_kernel<GatherSink<T, DIM>>), dim3((unsigned)blocks), dim3(kThreads),
-                                   0, stream, inK, inV, n, shift, w, blocks, ghist, oK,
-                                   (GatherSink<T, DIM>{g.P, g.pStride, g.pIndex, g.out, g.outStride}), skip);
+                launch_scatter(inK, inV, n, shift, w, blocks, ghist, oK,
+                               GatherSink<T, DIM>{g.P, g.pStride, g.pIndex, g.out, g.outStride}, skip, stream);
                 ok = true;
             });
         });
