@@ -164,6 +164,48 @@ int rdhip_batch_ridge_detection_dims(rdhip_ctx* ctx, int64_t num_clouds, const v
                                      const int64_t* dims, const int64_t* sizes, const double* r1,
                                      const double* r2, int64_t max_evolve_iters, int64_t* out_counts);
 
+/* ---- seeded synthetic samples on the device ------------------------------------------
+ * Replaces rd::gpu::SamplesGenerator<T> (rd/gpu/device/samples_generator.cuh:291-384:
+ * circle, sphere, spiral2D, spiral3D, segmentND, writing into a device buffer) with a
+ * counter-based generator (Philox4x32-10) keyed by `seed`: sample i of stream `stream_id`
+ * is the same whatever n, the launch shape or the other clouds.  Every coordinate gets
+ * N(0, sigma^2) noise (the reference scales its sigma by 1/sqrt(dim) first; pass the scaled
+ * value to reproduce its spread).  Kinds and their params: */
+enum rdhip_gen_kind {
+    RDHIP_GEN_CIRCLE = 0,    /* dim 2, {x0, y0, r}: p = 2 pi U, (x0 + r cos p, y0 + r sin p) */
+    RDHIP_GEN_SPHERE = 1,    /* dim 3, {x0, y0, z0, r}: p = pi U1, q = 2 pi U2,
+                                (x0 + r cos q sin p, y0 + r sin q sin p, z0 + r cos p) */
+    RDHIP_GEN_SPIRAL2D = 2,  /* dim 2, {a, b}: p = a U, (b p cos p, b p sin p) */
+    RDHIP_GEN_SPIRAL3D = 3,  /* dim 3, {a, b}: p = a U, (b p cos p, b p sin p, b p) */
+    RDHIP_GEN_SEGMENT = 4,   /* any dim, {length}: (length U, 0, ..., 0) */
+    RDHIP_GEN_HELIX = 5,     /* dim <= 3, {turns, pitch}: t = 2 pi turns U, first dim of (cos t, sin t, pitch t) */
+    RDHIP_GEN_POLYLINE = 6,  /* any dim, {m, v_0, ..., v_m (dim values each)}: s = floor(m U1),
+                                v_s + U2 (v_{s+1} - v_s) */
+    RDHIP_GEN_UNIFORM = 7    /* any dim, {}: every coordinate uniform in (0, 1], no noise */
+};
+
+/* n samples into `out` (device memory when out_on_device, else host memory staged through
+ * the device), ROW_MAJOR (row stride ld >= dim) or COL_MAJOR / SOA (column stride ld >= n;
+ * 0 = packed).  Synchronous. */
+int rdhip_generate(int dtype, int kind, int64_t n, int dim, const double* params, int64_t num_params,
+                   double sigma, uint64_t seed, uint32_t stream_id, void* out, int layout, int64_t ld,
+                   int out_on_device);
+
+/* The same straight into the context's point buffer (its dimension, stream 0): the device
+ * form of rdhip_set_points for generated clouds (the reference tests fill dP_ this way). */
+int rdhip_generate_points(rdhip_ctx* ctx, int kind, int64_t n, const double* params, int64_t num_params,
+                          double sigma, uint64_t seed);
+
+/* No reference counterpart: a batch of clouds generated on the device as the context's
+ * resident points (cloud c = stream c, dims[c] <= the context's dimension, zero-padded as in
+ * rdhip_batch_ridge_detection_dims; its params start at params[param_offsets[c]]), then
+ * run with rdhip_batch_run -- batched benchmarks without a host upload. */
+int rdhip_batch_generate(rdhip_ctx* ctx, int64_t num_clouds, const int* kinds, const int64_t* dims,
+                         const int64_t* sizes, const double* params, const int64_t* param_offsets,
+                         const double* sigmas, const uint64_t* seeds);
+int rdhip_batch_run(rdhip_ctx* ctx, const double* r1, const double* r2, int64_t max_evolve_iters,
+                    int64_t* out_counts);
+
 /* Tiled ridge detection (tree build, per-tile choose/evolve, local or global decimation,
  * optional end-phase refinement: decimate + evolve over all points). tiles_per_dim has dim
  * entries; max_tile_capacity: a tile with at least this many own points is split. */

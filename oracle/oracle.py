@@ -227,6 +227,23 @@ class Oracle:
                                                   C.c_int(len(sizes)), _p(sizes), _p(alive))
         return int(k), alive[:K].astype(bool)
 
+    def philox(self, ctr, key):
+        """Philox4x32-10 of a 4-word counter and a 2-word key (oracle only)."""
+        c = np.ascontiguousarray(ctr, dtype=np.uint32)
+        k = np.ascontiguousarray(key, dtype=np.uint32)
+        out = np.zeros(4, dtype=np.uint32)
+        self.lib.oracle_philox4x32_10(_p(c), _p(k), _p(out))
+        return out
+
+    def generate(self, kind, n, dim, params=(), sigma=0.0, seed=0, stream=0, dtype=np.float32):
+        """The seeded generator's n samples (oracle only), (n, dim) ROW_MAJOR."""
+        out = np.empty((max(int(n), 1), dim), dtype=dtype)
+        prm = np.ascontiguousarray(list(params) + [0.0], dtype=np.float64)
+        self._check(self._fn("generate", np.dtype(dtype))(C.c_int(kind), C.c_long(n), C.c_int(dim), _p(prm),
+                                                            C.c_double(sigma), C.c_uint64(seed), C.c_uint32(stream),
+                                                            _p(out)), "generate")
+        return out[:int(n)]
+
     def ridge_detection(self, P, r1, r2, max_evolve_iter=0, omp=False):
         P = np.ascontiguousarray(P)
         n, d = P.shape

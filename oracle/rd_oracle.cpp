@@ -18,7 +18,11 @@
 //     argmin for the evolve assignment, partition invariants for the trees, chain
 //     adjacency and coverage for the ordering) in tests/test_oracle_invariants.py;
 //   * the differentiated evolve (no reference counterpart) by the bit-exact replay of
-//     recorded runs and central finite differences (tests/test_oracle_vjp.py).
+//     recorded runs and central finite differences (tests/test_oracle_vjp.py);
+//   * the seeded generator (counterpart of rd/gpu/device/samples_generator.cuh with its own
+//     counter-based RNG): Philox4x32-10 by the published known-answer vectors of its
+//     authors, the shapes by their closed forms and sample statistics
+//     (tests/test_oracle_generate.py).
 //
 // Conventions: points are ROW_MAJOR (point i occupies P[i*dim .. i*dim+dim-1]).
 
@@ -970,7 +974,112 @@ long treeC(int kind, const T* P, long n, int dim, T r1, long cap, const long* tp
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------
+// a12: seeded generator.  Restates ridgedetection_amd's device generator (the counterpart
+// of rd/gpu/device/samples_generator.cuh:27-384 with its own RNG): Philox4x32-10 (Salmon,
+// Moraes, Dror, Shaw, SC'11) with counter (sample low, sample high, stream, block) and key
+// (seed low, seed high); uniforms in (0, 1] from 24 / 53 bits; Box-Muller normals.
+// ---------------------------------------------------------------------------------
+static void philox4x32_10(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline float genUniform(float, uint32_t a, uint32_t) { return (float)((a >> 8) + 1u) * 5.9604644775390625e-08f; }
+static inline double genUniform(double, uint32_t a, uint32_t b) {
+    const uint64_t m = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+    return (double)(m + 1u) * 1.1102230246251565404e-16;
+}
+
+template <typename T>
+static void generate(int kind, long n, int dim, const double* p, double sigmaD, uint64_t seed, uint32_t stream,
+                     T* out) {
+    const T kPi = T(3.1415926535897932384626433832795), sigma = (T)sigmaD;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (long i = 0; i < n; ++i) {
+        auto draw = [&](uint32_t block, uint32_t o[4]) {
+            const uint32_t c[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), stream, block};
+            philox4x32_10(c, key, o);
+        };
+        uint32_t b0[4];
+        draw(0, b0);
+        const T u1 = genUniform(T(0), b0[0], b0[1]), u2 = genUniform(T(0), b0[2], b0[3]);
+        T base[12] = {};
+        switch (kind) {
+            case 0: {  // circle
+                const T ph = T(2) * kPi * u1;
+                base[0] = (T)p[0] + (T)p[2] * std::cos(ph);
+                base[1] = (T)p[1] + (T)p[2] * std::sin(ph);
+                break;
+            }
+            case 1: {  // sphere
+                const T ph = kPi * u1, th = T(2) * kPi * u2;
+                base[0] = (T)p[0] + (T)p[3] * std::cos(th) * std::sin(ph);
+                base[1] = (T)p[1] + (T)p[3] * std::sin(th) * std::sin(ph);
+                base[2] = (T)p[2] + (T)p[3] * std::cos(ph);
+                break;
+            }
+            case 2:
+            case 3: {  // spirals
+                const T ph = (T)p[0] * u1, bp = (T)p[1] * ph;
+                base[0] = bp * std::cos(ph);
+                base[1] = bp * std::sin(ph);
+                base[2] = kind == 3 ? bp : T(0);
+                break;
+            }
+            case 4: base[0] = (T)p[0] * u1; break;  // segment
+            case 5: {  // helix
+                const T t = T(2) * kPi * (T)p[0] * u1;
+                base[0] = std::cos(t);
+                base[1] = std::sin(t);
+                base[2] = (T)p[1] * t;
+                break;
+            }
+            case 6: {  // polyline
+                const int m = (int)p[0];
+                const int sg = std::min((int)((T)m * u1), m - 1);
+                const double* v0 = p + 1 + (long)sg * dim;
+                const double* v1 = v0 + dim;
+                for (int d = 0; d < dim; ++d) base[d] = (T)v0[d] + u2 * ((T)v1[d] - (T)v0[d]);
+                break;
+            }
+            case 7: {  // uniform
+                base[0] = u1;
+                base[1] = u2;
+                for (int d = 2; d < dim; d += 2) {
+                    uint32_t b[4];
+                    draw((uint32_t)(d >> 1), b);
+                    base[d] = genUniform(T(0), b[0], b[1]);
+                    base[d + 1] = genUniform(T(0), b[2], b[3]);
+                }
+                break;
+            }
+            default: break;
+        }
+        if (kind != 7 && sigma != T(0))
+            for (int d = 0; d < dim; d += 2) {
+                uint32_t b[4];
+                draw(1u + (uint32_t)(d >> 1), b);
+                const T v1 = genUniform(T(0), b[0], b[1]), v2 = genUniform(T(0), b[2], b[3]);
+                const T r = std::sqrt(T(-2) * std::log(v1)), t = T(6.283185307179586476925286766559) * v2;
+                base[d] = base[d] + sigma * (r * std::cos(t));
+                base[d + 1] = base[d + 1] + sigma * (r * std::sin(t));
+            }
+        for (int d = 0; d < dim; ++d) out[i * dim + d] = base[d];
+    }
+}
+
 extern "C" {
+
+void oracle_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { philox4x32_10(ctr, key, out); }
 
 const char* oracle_last_error() { return g_err.c_str(); }
 
@@ -1039,6 +1148,14 @@ int oracle_num_threads() {
         return guarded([&]() -> long {                                                               \
             if (nA < 1) throw std::runtime_error("distance stats of an empty set");                 \
             oracle::distanceStats<T>(A, nA, B, nB, dim, out4);                                       \
+            return 0;                                                                                \
+        });                                                                                          \
+    }                                                                                                \
+    long oracle_generate_##SUF(int kind, long n, int dim, const double* params, double sigma,         \
+                               uint64_t seed, uint32_t stream, T* out) {                             \
+        return guarded([&]() -> long {                                                               \
+            if (dim < 1 || dim > 12) throw std::runtime_error("dimension out of range");           \
+            generate<T>(kind, n, dim, params, sigma, seed, stream, out);                             \
             return 0;                                                                                \
         });                                                                                          \
     }                                                                                                \

@@ -32,11 +32,13 @@ EXPORTS = [
     "rdhip_decimate", "rdhip_set_option", "rdhip_evolve_fixed", "rdhip_build_tree", "rdhip_get_tree",
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
     "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats", "rdhip_batch_ridge_detection_dims",
+    "rdhip_generate", "rdhip_generate_points", "rdhip_batch_generate", "rdhip_batch_run",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
 OPT_LOCAL_MAX_SAMPLES, OPT_LOCAL_MAX_WORK, OPT_SCAN_THREADS, OPT_AUTOTUNE, OPT_SEGMENTED = 6, 7, 8, 9, 10
 OPT_POSITION_COPY = 11
+GEN_CIRCLE, GEN_SPHERE, GEN_SPIRAL2D, GEN_SPIRAL3D, GEN_SEGMENT, GEN_HELIX, GEN_POLYLINE, GEN_UNIFORM = range(8)
 
 
 class Timers(C.Structure):
@@ -82,6 +84,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_get_tree": (i32, [vp, vp, vp, vp, vp]),
         "rdhip_batch_ridge_detection": (i32, [vp, i64, vp, vp, vp, vp, i64, vp]),
         "rdhip_batch_ridge_detection_dims": (i32, [vp, i64, vp, vp, vp, vp, vp, i64, vp]),
+        "rdhip_generate": (i32, [C.c_int, C.c_int, i64, C.c_int, vp, i64, C.c_double, C.c_uint64, C.c_uint32, vp,
+                                 C.c_int, i64, C.c_int]),
+        "rdhip_generate_points": (i32, [vp, C.c_int, i64, vp, i64, C.c_double, C.c_uint64]),
+        "rdhip_batch_generate": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
+        "rdhip_batch_run": (i32, [vp, vp, vp, i64, vp]),
         "rdhip_evolve_fwd": (i32, [vp, C.c_double, i64, i64]),
         "rdhip_order_chains": (i32, [vp, C.c_double, vp, vp, C.POINTER(i64)]),
         "rdhip_set_distances": (i32, [vp, vp, i64, vp, i64, vp]),
@@ -103,6 +110,21 @@ class RDError(RuntimeError):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def generate(kind, n, dim, params=(), sigma=0.0, seed=0, stream_id=0, dtype=np.float32):
+    """n seeded samples of a generator kind (GEN_*), computed on the GPU, as an (n, dim)
+    host array."""
+    lib = load()
+    dt = np.dtype(dtype)
+    out = np.empty((max(int(n), 1), dim), dtype=dt)
+    prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+    rc = lib.rdhip_generate(RDHIP_F32 if dt == np.float32 else RDHIP_F64, int(kind), int(n), int(dim),
+                            _ptr(prm) if prm.size else None, int(prm.size), float(sigma), int(seed), int(stream_id),
+                            _ptr(out), ROW_MAJOR, 0, 0)
+    if rc != 0:
+        raise RDError(f"rdhip error {rc}: {lib.rdhip_last_error().decode()}")
+    return out[:int(n)]
 
 
 class Context:
@@ -196,6 +218,45 @@ class Context:
         allp = self.points()
         out, o = [], 0
         for k, dc in zip(counts, dims):
+            out.append(np.ascontiguousarray(allp[o:o + k, :dc]))
+            o += k
+        return out
+
+    def generate_points(self, kind, n, params=(), sigma=0.0, seed=0):
+        """n generated samples of the context's dimension as its points (stream 0)."""
+        prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        self._check(self.lib.rdhip_generate_points(self.h, int(kind), int(n), _ptr(prm) if prm.size else None,
+                                                   int(prm.size), float(sigma), int(seed)))
+        self.n = int(n)
+
+    def batch_generate(self, kinds, dims, sizes, params, sigmas, seeds):
+        """Clouds generated on the device as the resident batch: params is a list of
+        per-cloud parameter sequences."""
+        m = len(kinds)
+        off = np.zeros(m, dtype=np.int64)
+        flat = []
+        for c, p in enumerate(params):
+            off[c] = len(flat)
+            flat.extend(float(x) for x in p)
+        prm = np.ascontiguousarray(flat if flat else [0.0], dtype=np.float64)
+        self._batch_dims = np.ascontiguousarray(dims, dtype=np.int64).reshape(m)
+        self._batch_sizes = np.ascontiguousarray(sizes, dtype=np.int64).reshape(m)
+        self._check(self.lib.rdhip_batch_generate(
+            self.h, m, _ptr(np.ascontiguousarray(kinds, dtype=np.int32)), _ptr(self._batch_dims),
+            _ptr(self._batch_sizes), _ptr(prm), _ptr(off), _ptr(np.ascontiguousarray(sigmas, dtype=np.float64)),
+            _ptr(np.ascontiguousarray(seeds, dtype=np.uint64))))
+        self.n = int(self._batch_sizes.sum())
+
+    def batch_run(self, r1, r2, max_evolve_iters=0):
+        """Ridge detection of the resident batch; per-cloud results (first dims[c] coordinates)."""
+        m = len(self._batch_sizes)
+        vr1 = np.ascontiguousarray(r1, dtype=np.float64).reshape(m)
+        vr2 = np.ascontiguousarray(r2, dtype=np.float64).reshape(m)
+        counts = np.zeros(m, dtype=np.int64)
+        self._check(self.lib.rdhip_batch_run(self.h, _ptr(vr1), _ptr(vr2), int(max_evolve_iters), _ptr(counts)))
+        allp = self.points()
+        out, o = [], 0
+        for k, dc in zip(counts, self._batch_dims):
             out.append(np.ascontiguousarray(allp[o:o + k, :dc]))
             o += k
         return out

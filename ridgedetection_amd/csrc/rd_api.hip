@@ -1,11 +1,13 @@
 // rd_api.hip -- the C ABI (include/rdhip.h) over the device engine.
 #include "../../include/rdhip.h"
 #include "rd_engine.hpp"
+#include "rd_generate.hpp"
 #include "rd_quality.hpp"
 #include "rd_tiled.hpp"
 #include "rd_tree.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -27,6 +29,7 @@ struct rdhip_ctx {
     std::unique_ptr<TileTreeBuilder<float>> tree32;
     std::unique_ptr<TileTreeBuilder<double>> tree64;
     int treeKind = -1;
+    std::vector<int64_t> batchSizes;   // clouds of the resident batch (rdhip_batch_generate)
 
     std::unique_ptr<TileTreeBuilder<float>>& tree(float) { return tree32; }
     std::unique_ptr<TileTreeBuilder<double>>& tree(double) { return tree64; }
@@ -127,6 +130,7 @@ int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, 
         e.setPoints(static_cast<const T*>(points), n, layout, stride, on_device != 0);
         ctx->hasPoints = true;
         ctx->hasChosen = false;
+        ctx->batchSizes.clear();
         return RDHIP_OK;
     });
 }
@@ -148,7 +152,48 @@ int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evol
     });
 }
 
+}  // extern "C"
+
 namespace {
+
+// Runs the batch whose clouds (sizes) are the context's resident points, in order.
+template <typename E, typename T>
+int run_batch(rdhip_ctx* ctx, E& e, T, const std::vector<int64_t>& sizes, const double* r1, const double* r2,
+              int64_t max_evolve_iters, int64_t* out_counts) {
+    const int64_t num = (int64_t)sizes.size();
+    std::vector<SegDesc> segs((size_t)num);
+    std::vector<int64_t> caps((size_t)num);
+    std::vector<T> vr1((size_t)num), vr2((size_t)num);
+    int64_t off = 0;
+    for (int64_t c = 0; c < num; ++c) {
+        SegDesc s{};
+        s.pOff = off;
+        s.nP = (int32_t)sizes[c];
+        s.nOwn = (int32_t)sizes[c];
+        s.group = (int32_t)c;
+        segs[c] = s;
+        caps[c] = sizes[c];
+        vr1[c] = (T)r1[c];
+        vr2[c] = (T)r2[c];
+        off += sizes[c];
+    }
+    e.setSegments(segs, caps, vr1, vr2, false);
+    RunOptions opt = e.options;
+    opt.maxEvolveIters = max_evolve_iters;
+    e.ridgeDetectionLocal(opt);
+    ctx->hasChosen = true;
+    if (out_counts) {
+        std::vector<int> K = e.countsHost();
+        for (int64_t c = 0; c < num; ++c) out_counts[c] = K[c];
+    }
+    return RDHIP_OK;
+}
+
+int check_batch_radii(int64_t num, const double* r1, const double* r2) {
+    for (int64_t c = 0; c < num; ++c)
+        if (!(r1[c] >= 0) || !(r2[c] >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
+    return RDHIP_OK;
+}
 
 // Batched ridge detection; dims (optional) gives every cloud's own dimension <= the
 // context's: its rows are padded with zero coordinates, which leave every distance (the
@@ -165,16 +210,13 @@ int batch_rd(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, cons
         int64_t total = 0;
         for (int64_t c = 0; c < num_clouds; ++c) {
             if (sizes[c] < 1 || !clouds[c]) return fail(RDHIP_ERR_INVALID, "every cloud needs at least one point");
-            if (!(r1[c] >= 0) || !(r2[c] >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
             if (dims && (dims[c] < 1 || dims[c] > dim))
                 return fail(RDHIP_ERR_INVALID, "cloud dimensions must be in [1, context dimension]");
             total += sizes[c];
         }
+        if (int rc = check_batch_radii(num_clouds, r1, r2)) return rc;
         if (total >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "batch exceeds 2^31 points");
         std::vector<T> all((size_t)total * dim, T(0));
-        std::vector<SegDesc> segs((size_t)num_clouds);
-        std::vector<int64_t> caps((size_t)num_clouds);
-        std::vector<T> vr1((size_t)num_clouds), vr2((size_t)num_clouds);
         int64_t off = 0;
         for (int64_t c = 0; c < num_clouds; ++c) {
             const int dc = dims ? (int)dims[c] : dim;
@@ -185,34 +227,65 @@ int batch_rd(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, cons
                 for (int64_t i = 0; i < sizes[c]; ++i)
                     std::memcpy(all.data() + (off + i) * dim, src + i * dc, sizeof(T) * dc);
             }
-            SegDesc s{};
-            s.pOff = off;
-            s.nP = (int32_t)sizes[c];
-            s.nOwn = (int32_t)sizes[c];
-            s.group = (int32_t)c;
-            segs[c] = s;
-            caps[c] = sizes[c];
-            vr1[c] = (T)r1[c];
-            vr2[c] = (T)r2[c];
             off += sizes[c];
         }
         e.timers = Timers();
         e.setPoints(all.data(), total, RDHIP_ROW_MAJOR, 0, false);
         ctx->hasPoints = true;
-        e.setSegments(segs, caps, vr1, vr2, false);
-        RunOptions opt = e.options;
-        opt.maxEvolveIters = max_evolve_iters;
-        e.ridgeDetectionLocal(opt);
-        ctx->hasChosen = true;
-        if (out_counts) {
-            std::vector<int> K = e.countsHost();
-            for (int64_t c = 0; c < num_clouds; ++c) out_counts[c] = K[c];
-        }
-        return RDHIP_OK;
+        ctx->batchSizes.clear();
+        return run_batch(ctx, e, zero, std::vector<int64_t>(sizes, sizes + num_clouds), r1, r2, max_evolve_iters,
+                         out_counts);
     });
 }
 
+// Clouds generated on the device into dst (SoA rows of stride ld, padded to padDim
+// coordinates), cloud c at position offsets[c]; params are the host parameter lists.
+template <typename T>
+void generate_clouds(hipStream_t st, T* dst, int64_t ld, int padDim, const std::vector<GenCloud<T>>& proto,
+                     const std::vector<double>& params, const std::vector<int64_t>& paramOff,
+                     const std::vector<int64_t>& offsets) {
+    const int num = (int)proto.size();
+    DevBuf<double> dParams;
+    dParams.reserve(std::max<size_t>(params.size(), 1));
+    if (!params.empty())
+        check(hipMemcpyAsync(dParams.get(), params.data(), sizeof(double) * params.size(), hipMemcpyHostToDevice, st),
+              "generator parameters");
+    std::vector<GenCloud<T>> gc(proto);
+    int64_t maxN = 0;
+    for (int c = 0; c < num; ++c) {
+        gc[c].out = dst + offsets[c];
+        gc[c].ld = ld;
+        gc[c].padDim = padDim;
+        gc[c].rowMajor = 0;
+        gc[c].params = dParams.get() + paramOff[c];
+        maxN = std::max(maxN, gc[c].n);
+    }
+    DevBuf<GenCloud<T>> dClouds;
+    dClouds.reserve(gc.size());
+    check(hipMemcpyAsync(dClouds.get(), gc.data(), sizeof(GenCloud<T>) * gc.size(), hipMemcpyHostToDevice, st),
+          "generator clouds");
+    for (int c0 = 0; c0 < num; c0 += 65535)
+        check(launch_generate<T>(dClouds.get() + c0, std::min(num - c0, 65535), maxN, st), "generate");
+    check(hipStreamSynchronize(st), "generate");
+}
+
+// Validates one generator request; returns its parameter count (or a negative error).
+int gen_request(int kind, int64_t n, int dim, const double* params, int64_t num_params, double sigma) {
+    if (n < 0) return fail(RDHIP_ERR_INVALID, "negative sample count");
+    if (!(sigma >= 0) || !std::isfinite(sigma)) return fail(RDHIP_ERR_INVALID, "sigma must be finite and >= 0");
+    const int kd = gen_kind_dim(kind);
+    if (kd != 0 && kd != dim) return fail(RDHIP_ERR_INVALID, "this generator kind has a fixed dimension");
+    const int np = gen_kind_params(kind, dim, params, num_params);
+    if (np < 0) return fail(RDHIP_ERR_INVALID, "unknown generator kind, bad dimension or bad parameters");
+    if (np > num_params || (np > 0 && !params)) return fail(RDHIP_ERR_INVALID, "too few generator parameters");
+    for (int k = 0; k < np; ++k)
+        if (!std::isfinite(params[k])) return fail(RDHIP_ERR_INVALID, "generator parameters must be finite");
+    return np;
+}
+
 }  // namespace
+
+extern "C" {
 
 int rdhip_batch_ridge_detection(rdhip_ctx* ctx, int64_t num_clouds, const void* const* clouds, const int64_t* sizes,
                                 const double* r1, const double* r2, int64_t max_evolve_iters, int64_t* out_counts) {
@@ -224,6 +297,118 @@ int rdhip_batch_ridge_detection_dims(rdhip_ctx* ctx, int64_t num_clouds, const v
                                      int64_t max_evolve_iters, int64_t* out_counts) {
     if (!dims) return fail(RDHIP_ERR_INVALID, "null dimension array");
     return batch_rd(ctx, num_clouds, clouds, dims, sizes, r1, r2, max_evolve_iters, out_counts);
+}
+
+int rdhip_generate(int dtype, int kind, int64_t n, int dim, const double* params, int64_t num_params, double sigma,
+                   uint64_t seed, uint32_t stream_id, void* out, int layout, int64_t ld, int out_on_device) {
+    if (dtype != RDHIP_F32 && dtype != RDHIP_F64) return fail(RDHIP_ERR_INVALID, "dtype must be RDHIP_F32 or RDHIP_F64");
+    const int np = gen_request(kind, n, dim, params, num_params, sigma);
+    if (np < 0) return np;
+    if (n > 0 && !out) return fail(RDHIP_ERR_INVALID, "null output");
+    if (layout < RDHIP_ROW_MAJOR || layout > RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "unknown layout");
+    const bool row = layout == RDHIP_ROW_MAJOR;
+    if (ld == 0) ld = row ? dim : n;
+    if (row ? ld < dim : ld < n) return fail(RDHIP_ERR_INVALID, "leading dimension too small");
+    if (n == 0) return RDHIP_OK;
+    auto body = [&](auto zero) -> int {
+        using T = decltype(zero);
+        hipStream_t st = nullptr;
+        check(hipStreamCreate(&st), "hipStreamCreate");
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{st};
+        const int64_t elems = row ? n * ld : (int64_t)dim * ld;
+        DevBuf<T> staging;
+        T* dst = static_cast<T*>(out);
+        if (!out_on_device) dst = staging.reserve((size_t)elems);
+        DevBuf<double> dParams;
+        dParams.reserve((size_t)std::max(np, 1));
+        if (np > 0)
+            check(hipMemcpyAsync(dParams.get(), params, sizeof(double) * np, hipMemcpyHostToDevice, st),
+                  "generator parameters");
+        GenCloud<T> g{kind, dim, dim, row ? 1 : 0, n, ld, dst, dParams.get(), sigma, seed, stream_id};
+        DevBuf<GenCloud<T>> dg;
+        dg.reserve(1);
+        check(hipMemcpyAsync(dg.get(), &g, sizeof(g), hipMemcpyHostToDevice, st), "generator cloud");
+        check(launch_generate<T>(dg.get(), 1, n, st), "generate");
+        if (!out_on_device)
+            check(hipMemcpyAsync(out, dst, sizeof(T) * elems, hipMemcpyDeviceToHost, st), "download");
+        check(hipStreamSynchronize(st), "generate");
+        return RDHIP_OK;
+    };
+    return guarded([&]() -> int { return dtype == RDHIP_F32 ? body(0.0f) : body(0.0); });
+}
+
+int rdhip_generate_points(rdhip_ctx* ctx, int kind, int64_t n, const double* params, int64_t num_params,
+                          double sigma, uint64_t seed) {
+    if (!ctx) return fail(RDHIP_ERR_INVALID, "null context");
+    if (n < 0 || n >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "point count out of range");
+    const int np = gen_request(kind, n, ctx->dim, params, num_params, sigma);
+    if (np < 0) return np;
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        T* P = e.reservePoints(n);
+        GenCloud<T> g{kind, ctx->dim, ctx->dim, 0, n, 0, nullptr, nullptr, sigma, seed, 0};
+        generate_clouds<T>(e.stream(), P, e.pointStride(), ctx->dim, {g},
+                           std::vector<double>(params, params + np), {0}, {0});
+        ctx->hasPoints = true;
+        ctx->hasChosen = false;
+        ctx->batchSizes.clear();
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_batch_generate(rdhip_ctx* ctx, int64_t num_clouds, const int* kinds, const int64_t* dims,
+                         const int64_t* sizes, const double* params, const int64_t* param_offsets,
+                         const double* sigmas, const uint64_t* seeds) {
+    if (!ctx) return fail(RDHIP_ERR_INVALID, "null context");
+    if (num_clouds < 1 || num_clouds > (int64_t(1) << 24)) return fail(RDHIP_ERR_INVALID, "cloud count out of range");
+    if (!kinds || !dims || !sizes || !param_offsets || !sigmas || !seeds)
+        return fail(RDHIP_ERR_INVALID, "null batch arrays");
+    int64_t total = 0;
+    std::vector<double> allParams;
+    std::vector<int64_t> paramOff((size_t)num_clouds), offsets((size_t)num_clouds);
+    for (int64_t c = 0; c < num_clouds; ++c) {
+        if (sizes[c] < 1) return fail(RDHIP_ERR_INVALID, "every cloud needs at least one point");
+        if (dims[c] < 1 || dims[c] > ctx->dim)
+            return fail(RDHIP_ERR_INVALID, "cloud dimensions must be in [1, context dimension]");
+        if (param_offsets[c] < 0) return fail(RDHIP_ERR_INVALID, "negative parameter offset");
+        const double* pc = params ? params + param_offsets[c] : nullptr;
+        const int np = gen_request(kinds[c], sizes[c], (int)dims[c], pc, pc ? (int64_t(1) << 30) : 0, sigmas[c]);
+        if (np < 0) return np;
+        paramOff[c] = (int64_t)allParams.size();
+        allParams.insert(allParams.end(), pc, pc + np);
+        offsets[c] = total;
+        total += sizes[c];
+    }
+    if (total >= (int64_t(1) << 31)) return fail(RDHIP_ERR_INVALID, "batch exceeds 2^31 points");
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        using T = decltype(zero);
+        std::vector<GenCloud<T>> proto((size_t)num_clouds);
+        for (int64_t c = 0; c < num_clouds; ++c)
+            proto[c] = GenCloud<T>{kinds[c], (int)dims[c], ctx->dim, 0, sizes[c], 0, nullptr, nullptr, sigmas[c],
+                                   seeds[c], (uint32_t)c};
+        T* P = e.reservePoints(total);
+        generate_clouds<T>(e.stream(), P, e.pointStride(), ctx->dim, proto, allParams, paramOff, offsets);
+        ctx->hasPoints = true;
+        ctx->hasChosen = false;
+        ctx->batchSizes.assign(sizes, sizes + num_clouds);
+        return RDHIP_OK;
+    });
+}
+
+int rdhip_batch_run(rdhip_ctx* ctx, const double* r1, const double* r2, int64_t max_evolve_iters,
+                    int64_t* out_counts) {
+    if (!ctx) return fail(RDHIP_ERR_INVALID, "null context");
+    if (ctx->batchSizes.empty()) return fail(RDHIP_ERR_STATE, "no batch generated on the device");
+    if (!r1 || !r2) return fail(RDHIP_ERR_INVALID, "null radii");
+    if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
+    if (int rc = check_batch_radii((int64_t)ctx->batchSizes.size(), r1, r2)) return rc;
+    return withEngine(ctx, [&](auto& e, auto zero) -> int {
+        e.timers = Timers();
+        return run_batch(ctx, e, zero, ctx->batchSizes, r1, r2, max_evolve_iters, out_counts);
+    });
 }
 
 int rdhip_tiled_ridge_detection(rdhip_ctx* ctx, int algorithm, double r1, double r2, int64_t max_tile_capacity,

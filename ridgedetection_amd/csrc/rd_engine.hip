@@ -104,6 +104,17 @@ Engine<T>::~Engine() {
 }
 
 template <typename T>
+T* Engine<T>::reservePoints(int64_t n) {
+    if (n < 0) throw std::invalid_argument("negative point count");
+    ++version_;
+    psegValid_ = false;
+    nPoints_ = n;
+    pStride_ = round_up(std::max<int64_t>(n, 1), 64);
+    P_.reserve((size_t)pStride_ * dim_);
+    return P_.get();
+}
+
+template <typename T>
 void Engine<T>::setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice) {
     if (n < 0) throw std::invalid_argument("negative point count");
     ++version_;

@@ -106,6 +106,8 @@ public:
     // Uploads n points (host or device pointer) given in ROW_MAJOR, COL_MAJOR (stride =
     // elements between rows of the same coordinate) or SOA layout into the SoA buffer.
     void setPoints(const T* src, int64_t n, int layout, int64_t stride, bool onDevice);
+    // point buffer for n points filled on the device by the caller (SoA, pointStride())
+    T* reservePoints(int64_t n);
     int64_t numPoints() const { return nPoints_; }
     const T* devicePoints() const { return P_.get(); }
     int64_t pointStride() const { return pStride_; }
