@@ -131,6 +131,57 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
             "batched_s": t_batch, "batched_by_dimension_s": t_split, "looped_s": t_loop}
 
 
+def batched_device_generated(count=512, seed=7):
+    """The batched workload's clouds (same sizes, kinds, noise fractions and radii rule as
+    synth.batched_clouds, drawn from the same seeded sequence) generated on the device as
+    the resident batch, then run: no host generation and no upload.  Extents (for sigma) come
+    from the noise-free curves' closed forms / polyline vertices."""
+    import math
+    from ridgedetection_amd import _lib as L
+    from ridgedetection_amd.synth import _rng
+    g = _rng(seed)
+    kinds, dims, sizes, params, sigmas, seeds, r1 = [], [], [], [], [], [], []
+    for i in range(count):
+        dim = 2 if i < count // 2 else 3
+        n = int(round(math.exp(g.uniform(math.log(2000), math.log(500000)))))
+        kind = int(g.integers(0, 3))
+        frac = float(g.uniform(0.01, 0.1))
+        s = int(g.integers(0, 2 ** 31))
+        if kind == 0 and dim == 2:      # spiral t in [0, 6 pi]: (0.5 t cos t, 0.5 t sin t)
+            kinds.append(L.GEN_SPIRAL2D)
+            params.append([6 * math.pi, 0.5])
+            t = np.linspace(0, 6 * math.pi, 20001)
+            ext = float(np.max(np.ptp(np.stack([0.5 * t * np.cos(t), 0.5 * t * np.sin(t)]), axis=1)))
+        elif kind in (0, 1):            # helix, 5 turns, pitch 0.15 (first dim coordinates)
+            kinds.append(L.GEN_HELIX)
+            params.append([5.0, 0.15])
+            ext = 2.0 if dim == 2 else max(2.0, 0.15 * 10 * math.pi)
+        else:                           # polyline through 5 vertices in [-1, 1]^dim
+            kinds.append(L.GEN_POLYLINE)
+            v = _rng(s).uniform(-1.0, 1.0, (5, dim))
+            params.append([4] + v.ravel().tolist())
+            ext = float(np.max(np.ptp(v, axis=0)))
+        sigma = frac * ext
+        dims.append(dim)
+        sizes.append(n)
+        sigmas.append(sigma)
+        seeds.append(s)
+        r1.append(4.0 * sigma)
+    r2 = [2.0 * r for r in r1]
+    with L.Context(3, np.float32) as c:
+        c.batch_generate(kinds[:8], dims[:8], [min(x, 4000) for x in sizes[:8]], params[:8], sigmas[:8], seeds[:8])
+        c.batch_run(r1[:8], r2[:8])     # warm-up
+        log("bench: device-generated batch")
+        t0 = time.perf_counter()
+        c.batch_generate(kinds, dims, sizes, params, sigmas, seeds)
+        t1 = time.perf_counter()
+        res = c.batch_run(r1, r2)
+        t2 = time.perf_counter()
+    log(f"bench: device-generated {count} clouds: generate {t1 - t0:.3f} s, run {t2 - t1:.2f} s")
+    return {"clouds": count, "points": int(sum(sizes)), "ridge_points": int(sum(len(x) for x in res)),
+            "generate_s": t1 - t0, "run_s": t2 - t1, "clouds_per_s": count / (t2 - t0)}
+
+
 def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
     """Differentiated evolve on the 200k-point helix (sigma 0.05, r 0.2): forward pass of
     `iters` iterations keeping checkpoints every `every` iterations, then the backward pass
@@ -373,6 +424,7 @@ def main():
     if not args.no_tiled and dt == np.float32:
         result["secondary"]["tiled"] = tiled_runs()
     if not args.no_batch:
+        result["secondary"]["batched_512_device_generated"] = batched_device_generated()
         result["secondary"]["batched_512_clouds"] = batched_vs_looped()
     if not args.no_cpu:
         log("bench: CPU baseline")
