@@ -9,7 +9,12 @@
 //   * errors throw rd::gpu::Error (the reference calls checkCudaErrors, which exits);
 //   * TileType gains RD_LOCAL_TILE and RD_GROUPED_TILE (the reference GPU only has
 //     RD_EXTENDED_TILE; the CPU reference has all three trees);
-//   * RD_KD_TREE / RD_BVH_TREE are declared but, as in the reference, not implemented.
+//   * RD_KD_TREE / RD_BVH_TREE are declared but, as in the reference, not implemented;
+//   * rd::gpu::SamplesGenerator<T> (ridge_detection/rd/gpu/device/samples_generator.cuh:
+//     291-384) draws from librdhip's counter-based generator (seed 1234, as the reference's
+//     curand_init), so its samples follow the reference's distributions but are not the
+//     reference's curand values; the T** (one pointer per coordinate) overloads are not
+//     provided.
 #ifndef RDHIP_RIDGE_DETECTION_HPP
 #define RDHIP_RIDGE_DETECTION_HPP
 
@@ -17,7 +22,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
 #include <cstddef>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -58,6 +65,44 @@ struct TiledRidgeDetectionTimers {
 
 // rd/gpu/device/dispatch/dispatch_ridge_detection.cuh:39-74
 enum RidgeDetectionAlgorithm { RD_BRUTE_FORCE, RD_KD_TREE, RD_BVH_TREE };
+
+// rd/gpu/device/samples_generator.cuh:291-384: noisy samples of simple shapes written into a
+// device buffer (ROW_MAJOR: n x dim; COL_MAJOR: dim rows of stride n).  As in the reference,
+// sigma is divided by sqrt(dim) so that a sample's distance to the shape has spread sigma.
+template <typename T>
+struct SamplesGenerator {
+    template <DataMemoryLayout SAMPLES_ORDER = ROW_MAJOR>
+    static void spiral2D(size_t n, T a, T b, T sigma, T* dSamples) {
+        gen(SAMPLES_ORDER, RDHIP_GEN_SPIRAL2D, n, 2, {(double)a, (double)b}, sigma, dSamples);
+    }
+    template <DataMemoryLayout SAMPLES_ORDER = ROW_MAJOR>
+    static void spiral3D(size_t n, T a, T b, T sigma, T* dSamples) {
+        gen(SAMPLES_ORDER, RDHIP_GEN_SPIRAL3D, n, 3, {(double)a, (double)b}, sigma, dSamples);
+    }
+    template <DataMemoryLayout SAMPLES_ORDER = ROW_MAJOR>
+    static void segmentND(size_t n, size_t dim, T sigma, T length, T* dSamples, int stride = 0) {
+        gen(SAMPLES_ORDER, RDHIP_GEN_SEGMENT, n, (int)dim, {(double)length}, sigma, dSamples,
+            SAMPLES_ORDER == ROW_MAJOR ? 0 : stride);
+    }
+    template <DataMemoryLayout SAMPLES_ORDER = ROW_MAJOR>
+    static void circle(size_t n, T x0, T y0, T r, T sigma, T* dSamples) {
+        gen(SAMPLES_ORDER, RDHIP_GEN_CIRCLE, n, 2, {(double)x0, (double)y0, (double)r}, sigma, dSamples);
+    }
+    template <DataMemoryLayout SAMPLES_ORDER = ROW_MAJOR>
+    static void sphere(size_t n, T x0, T y0, T z0, T r, T sigma, T* dSamples) {
+        gen(SAMPLES_ORDER, RDHIP_GEN_SPHERE, n, 3, {(double)x0, (double)y0, (double)z0, (double)r}, sigma,
+            dSamples);
+    }
+
+private:
+    static void gen(DataMemoryLayout order, int kind, size_t n, int dim, std::initializer_list<double> params,
+                    T sigma, T* dSamples, int64_t ld = 0) {
+        const double s = (double)sigma / std::sqrt((double)dim);
+        detail::check(rdhip_generate(detail::dtype<T>(), kind, (int64_t)n, dim, params.begin(), (int64_t)params.size(),
+                                     s, 1234, 0, dSamples, detail::layout(order), ld, 1),
+                      "SamplesGenerator");
+    }
+};
 
 namespace bruteForce {
 

@@ -3,7 +3,10 @@
 // rd::gpu::tiled::RidgeDetection and rd::gpu::bruteForce::RidgeDetection.
 // Usage: dropin_usage <input.bin> <output-prefix>
 //   input.bin: int64 n, int64 dim(=3), double r1, double r2, then n*3 float ROW_MAJOR points.
-//   writes <prefix>_brute.bin and <prefix>_tiled.bin: int64 count, then count*3 floats.
+//   writes <prefix>_brute.bin and <prefix>_tiled.bin: int64 count, then count*3 floats;
+//   then, as the reference's generator-driven tests do, fills dP_ with
+//   SamplesGenerator<float>::spiral3D (20000 samples, a 4 pi, b 0.3, sigma 0.05) and writes
+//   the samples (<prefix>_genpts.bin) and their ridge points (<prefix>_genrd.bin).
 #include "rdhip/ridge_detection.hpp"
 
 #include <cstdio>
@@ -51,6 +54,18 @@ int main(int argc, char** argv) {
         rdGpu.getChosenPoints(T.data());
         write((std::string(argv[2]) + "_tiled.bin").c_str(), T, k);
         std::printf("brute %zu tiled %d\n", bf.ns_, k);
+
+        // samples generated straight into the device buffer
+        const size_t ng = 20000;
+        rd::gpu::bruteForce::RidgeDetection<float, 3, rd::ROW_MAJOR, rd::ROW_MAJOR> bg(ng, 0.3f, 0.6f);
+        rd::gpu::SamplesGenerator<float>::spiral3D(ng, 4.0f * 3.14159265358979f, 0.3f, 0.05f, bg.dP_);
+        std::vector<float> G(ng * 3);
+        if (hipMemcpy(G.data(), bg.dP_, sizeof(float) * G.size(), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+        write((std::string(argv[2]) + "_genpts.bin").c_str(), G, (long long)ng);
+        bg.ridgeDetection();
+        std::vector<float> R(bg.ns_ * 3);
+        if (hipMemcpy(R.data(), bg.dS_, sizeof(float) * R.size(), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+        write((std::string(argv[2]) + "_genrd.bin").c_str(), R, (long long)bg.ns_);
     } catch (const rd::gpu::Error& e) {
         std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
         return 1;
