@@ -30,6 +30,15 @@ __global__ void fill_int_kernel(int* __restrict__ a, int64_t n, int v, const int
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = v;
 }
 
+// a[i] = b[i] = 0 for i < n (one launch for the chain bounds of an iteration)
+__global__ void zero_pair_kernel(int* __restrict__ a, int* __restrict__ b, int64_t n, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+}
+
 // ROW_MAJOR (n x dim, row stride `stride`) -> SoA (dim x pStride)
 template <typename T>
 __global__ void row_to_soa_kernel(const T* __restrict__ src, int64_t n, int dim, int64_t stride,
@@ -550,8 +559,8 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
                                 totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
                                 sortWs_.get(), st_, flags_.get(), sg),
           "sort");
-    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), kb, 0, flags_.get());
-    hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainEnd_.get(), kb, 0, flags_.get());
+    hipLaunchKernelGGL(zero_pair_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), chainEnd_.get(), kb,
+                       flags_.get());
     const int longShift = long_chain_shift(dim_, totalP);
     check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
                               longShift, flags_.get(), st_),

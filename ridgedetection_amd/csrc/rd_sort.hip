@@ -2,7 +2,7 @@
 //
 // LSD radix sort with digits of at most 11 bits.  Each pass is three launches:
 //   hist    -- per-block digit histogram (LDS atomics; order-free integer counts)
-//   scan    -- exclusive scan of the digit-major histogram [digit][block] (3 kernels)
+//   scan    -- exclusive scan of the digit-major histogram [digit][block] (2 kernels)
 //   scatter -- per-block stable ranking: every wave ranks its 64 keys with a ballot
 //              match (one ballot per digit bit), carries per-digit counters across its
 //              chunks in LDS, and the four waves of a block are stitched together by a
@@ -76,38 +76,25 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const int* __rest
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-// Phase 2: one block scans the partials in place and writes the grand total.
-__global__ __launch_bounds__(1024) void scan_partials_kernel(int* __restrict__ partials, int np,
-                                                             int64_t* __restrict__ total,
-                                                             const int* __restrict__ skip) {
-    __shared__ int buf[1024];
-    if (skip && *skip) return;
-    int carry = 0;
-    for (int b0 = 0; b0 < np; b0 += 1024) {
-        int i = b0 + threadIdx.x;
-        int v = i < np ? partials[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < np) partials[i] = carry + buf[threadIdx.x] - v;
-        int blockTotal = buf[1023];
-        __syncthreads();
-        carry += blockTotal;
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
-}
-
-// Phase 3: scan inside each block (16 consecutive entries per thread) plus its offset.
+// Phase 2: scan inside each block (16 consecutive entries per thread) plus its offset, the
+// sum of the partials of the blocks before it (summed here: a separate scan of the partials
+// would cost a launch); the last block writes the grand total.
 __global__ __launch_bounds__(kThreads) void scan_apply_kernel(int* __restrict__ a, int64_t L,
                                                               const int* __restrict__ partials,
+                                                              int64_t* __restrict__ total,
                                                               const int* __restrict__ skip) {
     __shared__ int buf[kThreads];
     if (skip && *skip) return;
+    int c = 0;
+    for (int q = threadIdx.x; q < (int)blockIdx.x; q += kThreads) c += partials[q];
+    buf[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) buf[threadIdx.x] += buf[threadIdx.x + o];
+        __syncthreads();
+    }
+    const int carry = buf[0];
+    __syncthreads();
     constexpr int per = kScanElems / kThreads;
     const int64_t b0 = (int64_t)blockIdx.x * kScanElems + (int64_t)threadIdx.x * per;
     int v[per];
@@ -126,13 +113,14 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(int* __restrict__ 
         buf[threadIdx.x] += t;
         __syncthreads();
     }
-    int run = partials[blockIdx.x] + buf[threadIdx.x] - s;
+    int run = carry + buf[threadIdx.x] - s;
 #pragma unroll
     for (int j = 0; j < per; ++j) {
         int64_t i = b0 + j;
         if (i < L) a[i] = run;
         run += v[j];
     }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kThreads - 1) *total = run;
 }
 
 // Where a ranked pair goes: its value into valsOut ...
@@ -333,10 +321,8 @@ hipError_t exclusive_scan_inplace(int* a, int64_t L, int* partials, int64_t* tot
     const int64_t scanBlocks = cdiv(L, kScanElems);
     hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream, a, L, partials,
                        nullptr);
-    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, stream, partials, (int)scanBlocks, total,
-                       nullptr);
     hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream, a, L, partials,
-                       nullptr);
+                       total, nullptr);
     return hipGetLastError();
 }
 
@@ -385,10 +371,8 @@ hipError_t sort_passes(const int* keysIn, const int* valsIn, const int64_t* dN, 
                            shift, w, (int)blocks, ghist, skip);
         hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream,
                            ghist, L, partials, skip);
-        hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, stream, partials,
-                           (int)scanBlocks, p == 0 ? dM : nullptr, skip);
         hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)scanBlocks), dim3(kThreads), 0, stream,
-                           ghist, L, partials, skip);
+                           ghist, L, partials, p == 0 ? dM : nullptr, skip);
         if (p == passes - 1)
             lastScatter(inK, inV, n, shift, w, (int)blocks, ghist, oK, oV);
         else
