@@ -12,7 +12,8 @@
 //              and writes them out in that order: consecutive lanes write consecutive
 //              addresses inside each digit's run instead of 64 scattered runs per store.
 //              Wider digits leave runs too short to gain, and the larger LDS footprint
-//              would halve the blocks per CU, so they write straight from the ranks.
+//              halves the blocks per CU, which costs more than it saves in launches of
+//              more than kStageMaxBlocks blocks: those write straight from the ranks.
 // Negative keys are dropped by the first pass, which also writes the count of kept
 // pairs.  Used to list the members of every chosen point in point order (evolve) and to
 // bin points by tile (tree build).
@@ -144,6 +145,7 @@ struct GatherSink {
 };
 
 constexpr int kStageBits = 9;
+constexpr int kStageMaxBlocks = 4096;   // larger launches keep the smaller LDS footprint
 
 template <bool kStage, typename Sink>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
@@ -304,7 +306,7 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 template <typename Sink>
 void launch_scatter(const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks, const int* ghist,
                     int* oK, const Sink& sink, const int* skip, hipStream_t stream) {
-    if (w <= kStageBits)
+    if (w <= kStageBits && blocks <= kStageMaxBlocks)
         hipLaunchKernelGGL((radix_scatter_kernel<true, Sink>), dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
                            inV, n, shift, w, blocks, ghist, oK, sink, skip);
     else
