@@ -69,7 +69,7 @@ def cpu_baseline(P, S, r, budget_s=24.0):
                       f"{el:.1f} s on {threads} host threads"}
 
 
-def batched_vs_looped(count=512, loop_every=8, seed=7):
+def batched_vs_looped(count=512, loop_every=8, seed=7, graph=True, batch_only=False):
     """512 clouds (256 with d=2, 256 with d=3), sizes log-uniform in [2000, 500000], each
     a noisy spiral / helix / segment chain with r1 = 4 sigma, r2 = 2 r1: one batch of all
     clouds (one launch per phase; the 2-D clouds run zero-padded to 3 coordinates, which
@@ -87,6 +87,10 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
         by_dim.setdefault(p.shape[1], []).append(i)
     dmax = max(by_dim)
     ctxs = {d: Context(d, np.float32) for d in by_dim}
+    if not graph:
+        from ridgedetection_amd._lib import OPT_GRAPH
+        for c in ctxs.values():
+            c.set_option(OPT_GRAPH, 0)
 
     def run(ctx, idx):
         res = ctx.batch([clouds[i][0] for i in idx], [clouds[i][1] for i in idx], [clouds[i][2] for i in idx])
@@ -101,6 +105,10 @@ def batched_vs_looped(count=512, loop_every=8, seed=7):
     out_counts = run(ctxs[dmax], list(range(count)))
     t_batch = time.perf_counter() - t0
     log(f"bench: batched {count} clouds in {t_batch:.2f} s")
+    if batch_only:
+        for c in ctxs.values():
+            c.close()
+        return {"clouds": count, "points": total_pts, "ridge_points": out_counts, "batched_s": t_batch}
     t0 = time.perf_counter()
     split_counts = sum(run(ctxs[d], idx) for d, idx in by_dim.items())
     t_split = time.perf_counter() - t0
@@ -329,7 +337,9 @@ def main():
     ap.add_argument("--tiled-case", action="append", default=[], metavar="WORKLOAD:ALG",
                     help="only run these tiled cases (helix|bezier6 : algorithm id), print them and exit")
     ap.add_argument("--no-graph", action="store_true",
-                    help="tiled cases with eager launches instead of graph replay (kernel traces)")
+                    help="tiled cases / the batch with eager launches instead of graph replay (kernel traces)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the 512-cloud batch (one mixed-dimension batch), print it and exit")
     ap.add_argument("--batch-cloud", type=int, default=-1, metavar="IDX",
                     help="only run cloud IDX of the 512-cloud batch on its own (profiling) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -340,6 +350,9 @@ def main():
         return
     if args.batch_cloud >= 0:
         batch_cloud(args.batch_cloud)
+        return
+    if args.batch_only:
+        print(json.dumps(batched_vs_looped(graph=not args.no_graph, batch_only=True)), flush=True)
         return
     if args.tiled_case:
         for case in args.tiled_case:
