@@ -104,9 +104,111 @@ __device__ __forceinline__ void shift_coordinate(const ShiftArgs<T>& a, int d, i
     if (moved) a.segMoved[seg] = 1;
 }
 
+// ---------------------------------------------------------------------------------
+// 4b'. ordered sums through LDS: a group of kLanes lanes (a whole wave, or a 16-lane row)
+// sums one chain.  Tiles of kLanes * kLongPer members are loaded coalesced one tile before
+// their turn (the loads of tile t + 1 are in flight while tile t is summed), staged in the
+// group's LDS slice and added in member order by every lane of the group from broadcast
+// LDS reads, so each member costs one plain dependent add (6.25 cycles, against 13.25 for
+// an add fed by a DPP row broadcast).  Members past the end of the chain are staged as +0
+// (see above).  The four rows of a wave read four slices whose starts differ by 16 bytes
+// modulo the bank width, so their broadcasts do not collide.
+// ---------------------------------------------------------------------------------
+constexpr int kLongPer = 16;
+constexpr int kLongTile = kLongPer * kWave;
+constexpr int kRowTile = kLongPer * 16;
+
+template <typename T>
+struct WideOf;
+template <>
+struct WideOf<float> { using type = float4; static constexpr int n = 4; };
+template <>
+struct WideOf<double> { using type = double2; static constexpr int n = 2; };
+
+// LDS per wave: one wave slice, or four row slices padded by one wide read each
+template <typename T>
+constexpr int wave_lds_elems() {
+    return kLongTile > 4 * (kRowTile + WideOf<T>::n) ? kLongTile : 4 * (kRowTile + WideOf<T>::n);
+}
+
+// sum + v.x + v.y (+ v.z + v.w), in that order
+__device__ __forceinline__ float add_wide(float sum, float4 v) {
+    sum = sum + v.x;
+    sum = sum + v.y;
+    sum = sum + v.z;
+    return sum + v.w;
+}
+__device__ __forceinline__ double add_wide(double sum, double2 v) {
+    sum = sum + v.x;
+    return sum + v.y;
+}
+
+// orders a wave's LDS accesses (the slices are private to the wave)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Serial sum of src[0 .. cnt) by a group of kLanes lanes (every lane of the wave calls it;
+// `steps` is the wave-uniform largest cnt of its groups); tile is the group's LDS slice.
+template <typename T, int kLanes>
+__device__ __forceinline__ T group_ordered_sum(const T* __restrict__ src, int cnt, int steps, T* tile) {
+    using W = typename WideOf<T>::type;
+    constexpr int kW = WideOf<T>::n;
+    constexpr int kTile = kLanes * kLongPer;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int nTiles = (steps + kTile - 1) / kTile;
+    const int last = max(cnt - 1, 0);
+    T reg[kLongPer];
+    // index clamped into the chain and masked (tile t + 1 is loaded after tile t is staged;
+    // its loads are not waited for before tile t's sum)
+    auto load = [&](int t) {
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) {
+            const int i = t * kTile + c * kLanes + lane;
+            const T x = src[min(i, last)];
+            reg[c] = i < cnt ? x : T(0);
+        }
+    };
+    load(0);
+    T sum = T(0);
+    for (int t = 0; t < nTiles; ++t) {
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) tile[c * kLanes + lane] = reg[c];
+        wave_lds_sync();
+        load(t + 1);
+        // groups of kG wide reads: group g + 1 is read while group g is added (the first
+        // pass adds a group of zeros, which leaves the sum unchanged); the scheduling
+        // barriers keep the reads a whole group ahead of their adds
+        const W* w = reinterpret_cast<const W*>(tile);
+        constexpr int kG = 8, kNQ = kTile / kW;
+        static_assert(kNQ % kG == 0, "tile is not a whole number of read groups");
+        W cur[kG];
+#pragma unroll
+        for (int g = 0; g < kG; ++g) cur[g] = W{};
+#pragma unroll 2
+        for (int q0 = 0; q0 < kNQ; q0 += kG) {
+            W nxt[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) nxt[g] = w[q0 + g];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) cur[g] = nxt[g];
+        }
+#pragma unroll
+        for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
+    }
+    return sum;
+}
+
 // 16-lane row path: one row per (coordinate, chosen slot) of chains not on the long list
 template <typename T, int DIM>
-__device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t blk) {
+__device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t blk, T* waveLds) {
     constexpr int kRow = 16;
     const int t = threadIdx.x & (kRow - 1);
     const int64_t row = blk * (256 / kRow) + threadIdx.x / kRow;
@@ -129,106 +231,15 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
             seg = -1;
         }
     }
-    // the wave steps through the longest of its four chains; DPP needs every lane present.
-    // Loads are unconditional (index clamped into the chain, or element 0 of the buffer for
-    // an empty one) so that no branch separates them from the folds.
+    // the wave steps through the longest of its four chains; every lane takes part in the
+    // staging.  Loads are unconditional (index clamped into the chain, or element 0 of the
+    // buffer for an empty one).
     const int steps = wave_row_max(cnt);
     const T* __restrict__ src = a.sortedP + (int64_t)min(d, DIM - 1) * a.mStride + off;
-    const int last = max(cnt - 1, 0);
-    const T sum = row_ordered_sum<T>(steps, [&](int i) {
-        const T x = src[min(i, last)];
-        return i < cnt ? x : T(0);
-    });
+    const int rowInWave = (threadIdx.x & (kWave - 1)) / kRow;
+    const T sum = group_ordered_sum<T, kRow>(src, cnt, steps, waveLds + rowInWave * (kRowTile + WideOf<T>::n));
     if (seg < 0 || t != 0) return;
     shift_coordinate(a, d, slot, seg, cnt, sum);
-}
-
-// ---------------------------------------------------------------------------------
-// 4c. ordered sums of the long chains: one wave per (coordinate, chosen point).  Tiles of
-// kLongTile members are loaded coalesced one tile before their turn (the loads of tile
-// t + 1 are in flight while tile t is summed), staged in LDS and added in member order by
-// every lane from broadcast LDS reads, so each member costs one dependent add.  Members
-// past the end of the chain are staged as +0 (see above).
-// ---------------------------------------------------------------------------------
-constexpr int kLongPer = 16;
-constexpr int kLongTile = kLongPer * kWave;
-
-template <typename T>
-struct WideOf;
-template <>
-struct WideOf<float> { using type = float4; static constexpr int n = 4; };
-template <>
-struct WideOf<double> { using type = double2; static constexpr int n = 2; };
-
-// sum + v.x + v.y (+ v.z + v.w), in that order
-__device__ __forceinline__ float add_wide(float sum, float4 v) {
-    sum = sum + v.x;
-    sum = sum + v.y;
-    sum = sum + v.z;
-    return sum + v.w;
-}
-__device__ __forceinline__ double add_wide(double sum, double2 v) {
-    sum = sum + v.x;
-    return sum + v.y;
-}
-
-// orders a wave's LDS accesses (the tile is private to the wave)
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_ordered_sum(const T* __restrict__ src, int cnt, T* tile) {
-    using W = typename WideOf<T>::type;
-    constexpr int kW = WideOf<T>::n;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int nTiles = (cnt + kLongTile - 1) / kLongTile;
-    const int last = max(cnt - 1, 0);
-    T reg[kLongPer];
-    // index clamped into the chain and masked (tile t + 1 is loaded after tile t is staged;
-    // its loads are not waited for before tile t's sum)
-    auto load = [&](int t) {
-#pragma unroll
-        for (int c = 0; c < kLongPer; ++c) {
-            const int i = t * kLongTile + c * kWave + lane;
-            const T x = src[min(i, last)];
-            reg[c] = i < cnt ? x : T(0);
-        }
-    };
-    load(0);
-    T sum = T(0);
-    for (int t = 0; t < nTiles; ++t) {
-        wave_lds_sync();
-#pragma unroll
-        for (int c = 0; c < kLongPer; ++c) tile[c * kWave + lane] = reg[c];
-        wave_lds_sync();
-        load(t + 1);
-        // groups of kG wide reads: group g + 1 is read while group g is added (the first
-        // pass adds a group of zeros, which leaves the sum unchanged); the scheduling
-        // barriers keep the reads a whole group ahead of their adds
-        const W* w = reinterpret_cast<const W*>(tile);
-        constexpr int kG = 8, kNQ = kLongTile / kW;
-        W cur[kG];
-#pragma unroll
-        for (int g = 0; g < kG; ++g) cur[g] = W{};
-#pragma unroll 2
-        for (int q0 = 0; q0 < kNQ; q0 += kG) {
-            W nxt[kG];
-#pragma unroll
-            for (int g = 0; g < kG; ++g) nxt[g] = w[q0 + g];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int g = 0; g < kG; ++g) cur[g] = nxt[g];
-        }
-#pragma unroll
-        for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
-    }
-    return sum;
 }
 
 // wave path: one wave per (list entry, coordinate)
@@ -243,17 +254,17 @@ __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wa
     if (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg]) return;
     const int off = a.chainOff[slot];
     const int cnt = a.chainEnd[slot] - off;
-    const T sum = wave_ordered_sum<T>(a.sortedP + (int64_t)d * a.mStride + off, cnt, tile);
+    const T sum = group_ordered_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt, cnt, tile);
     if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
 }
 
 // blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
-    __shared__ __align__(16) T tiles[256 / kWave][kLongTile];
+    __shared__ __align__(16) T tiles[256 / kWave][wave_lds_elems<T>()];
     if (*a.allDone) return;
     if ((int64_t)blockIdx.x < rowBlocks) {
-        chain_sum_rows<T, DIM>(a, blockIdx.x);
+        chain_sum_rows<T, DIM>(a, blockIdx.x, tiles[threadIdx.x / kWave]);
     } else {
         const int w = threadIdx.x / kWave;
         chain_sum_wave<T, DIM>(a, ((int64_t)blockIdx.x - rowBlocks) * (256 / kWave) + w, tiles[w]);
