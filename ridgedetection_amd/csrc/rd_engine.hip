@@ -559,12 +559,20 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
                                 totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
                                 sortWs_.get(), st_, flags_.get(), sg),
           "sort");
-    hipLaunchKernelGGL(zero_pair_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), chainEnd_.get(), kb,
-                       flags_.get());
     const int longShift = long_chain_shift(dim_, totalP);
-    check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
-                              longShift, flags_.get(), st_),
-          "chain_bounds");
+    const int sortKeys = (int)std::max<int64_t>(kb, 2);
+    if (sort_single_pass(sortKeys)) {
+        // every chosen slot's member run straight from the one-pass sort's scanned histogram
+        check(launch_key_ranges_from_sort(sortWs_.get(), totalP, sortKeys, dM_.get(), chainOff_.get(),
+                                          chainEnd_.get(), longList_.get(), longShift, flags_.get(), st_),
+              "key ranges");
+    } else {
+        hipLaunchKernelGGL(zero_pair_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), chainEnd_.get(),
+                           kb, flags_.get());
+        check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
+                                  longShift, flags_.get(), st_),
+              "chain_bounds");
+    }
     ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),

@@ -328,6 +328,43 @@ hipError_t exclusive_scan_inplace(int* a, int64_t L, int* partials, int64_t* tot
     return hipGetLastError();
 }
 
+bool sort_single_pass(int numKeys) { return numKeys <= (1 << kMaxDigitBits); }
+
+namespace {
+__global__ __launch_bounds__(256) void key_ranges_kernel(const int* __restrict__ ghist, int numBlocks,
+                                                         int numDigits, int numKeys, const int64_t* __restrict__ dM,
+                                                         int* __restrict__ off, int* __restrict__ end,
+                                                         int* __restrict__ longList, int longShift,
+                                                         const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= numKeys) return;
+    const int64_t M = *dM;
+    const int64_t o = ghist[(int64_t)k * numBlocks];
+    const int64_t e = k + 1 < numDigits ? (int64_t)ghist[(int64_t)(k + 1) * numBlocks] : M;
+    off[k] = (int)o;
+    end[k] = (int)e;
+    if (longList && e > o) {
+        const int64_t L = int64_t(1) << longShift;
+        const int64_t m0 = (o + L - 1) >> longShift;
+        for (int64_t m = m0; (m << longShift) < e; ++m) longList[m] = m == m0 ? k : -1;
+    }
+}
+}  // namespace
+
+hipError_t launch_key_ranges_from_sort(const void* workspace, int64_t maxN, int numKeys, const int64_t* dM,
+                                       int* off, int* end, int* longList, int longShift, const int* skip,
+                                       hipStream_t st) {
+    if (!sort_single_pass(numKeys)) return hipErrorInvalidValue;
+    int bits = 1;
+    while ((int64_t(1) << bits) < numKeys) ++bits;
+    const int64_t blocks = cdiv(std::max<int64_t>(maxN, 1), kBlockElems);
+    hipLaunchKernelGGL(key_ranges_kernel, dim3((unsigned)((numKeys + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const int*>(workspace), (int)blocks, 1 << bits, numKeys, dM, off, end, longList,
+                       longShift, skip);
+    return hipGetLastError();
+}
+
 size_t sort_workspace_bytes(int64_t maxN, int numKeys) {
     int bits = 1;
     while ((int64_t(1) << bits) < numKeys) ++bits;

@@ -8,6 +8,17 @@ namespace rdhip {
 // Workspace needed by stable_sort_by_key for up to maxN pairs with keys in [0, numKeys).
 size_t sort_workspace_bytes(int64_t maxN, int numKeys);
 
+// True when a sort over keys in [0, numKeys) runs in one pass (numKeys <= 2^11); its scanned
+// digit histogram then holds every key's output range.
+bool sort_single_pass(int numKeys);
+// After such a one-pass stable_sort_by_key (same workspace, maxN, numKeys; *dM pairs kept):
+// off[k] / end[k] = the range of key k in the sorted output for every k < numKeys, and,
+// when longList is given, longList[m] = the key whose range first holds position m 2^longShift
+// (-1 for the other multiples below *dM).  Returns hipErrorInvalidValue for a multi-pass sort.
+hipError_t launch_key_ranges_from_sort(const void* workspace, int64_t maxN, int numKeys, const int64_t* dM,
+                                       int* off, int* end, int* longList, int longShift, const int* skip,
+                                       hipStream_t st);
+
 // Sorts the *dN pairs (keysIn[i], valsIn ? valsIn[i] : i) by key, keeping input order
 // among equal keys; pairs with a negative key are dropped.  The number of kept pairs is
 // written to *dM.  All counts live on the device, so the call is stream-ordered and
