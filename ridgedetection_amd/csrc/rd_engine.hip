@@ -97,8 +97,8 @@ __global__ void relayout_slots_kernel(const T* __restrict__ S, int64_t sStride, 
 template <typename T>
 Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
     if (dim < 1 || dim > kMaxDim) throw std::invalid_argument("dimension must be in [1, 12]");
-    flags_.reserve(8);   // [0] all done, [1] iterations, [2] overflow, [3] queue, [4] active items,
-                         // [5] [6] local evolve
+    flags_.reserve(10);  // [0] all done, [1] iterations, [2] overflow, [3] queue, [4] active items,
+                         // [5] [6] local evolve, [8] blocks done (fused convergence bookkeeping)
     dN_.reserve(1);
     dM_.reserve(1);
 }
@@ -511,7 +511,7 @@ void Engine<T>::choose() {
 // evolve
 // ------------------------------------------------------------------------------------
 template <typename T>
-void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
+void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool finalize, int capIter) {
     const EvolveSource es = evolveSource();
     const bool ev = options.profile && slot >= 0;
     if (ev) {
@@ -533,6 +533,10 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
                         dSegR1Sqr_.get(), nullptr, (int)segs_.size(), segKStride_, segChunk_, 0, 0, nullptr, nullptr,
                         nullptr, labels_.get(), dSegDone_.get(), dSegMoved_.get(), flags_.get()};
         check(launch_segment_sums<T>(la, dim_, (int)segs_.size(), st_), "segment_sums");
+        if (finalize)
+            check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(),
+                                         capIter, flags_.get(), flags_.get() + 1, st_),
+                  "finalize");
         if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
         return;
     }
@@ -584,7 +588,19 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log) {
         ha.totalSlots = numRowSlots_;
         ha.rowSlots = rowSlots_.get();
     }
+    // A single cloud: the sum launch's last block does the convergence bookkeeping (one
+    // launch fewer).  Segment sets keep the separate launch: the device-scope fence every
+    // block of the fused form issues writes back the L2's dirty lines, which after the
+    // member scatter of a large batch costs more than the launch.
+    const bool fuseFinalize = finalize && segs_.size() == 1;
+    if (fuseFinalize)
+        ha.fin = {1, (int)segs_.size(), dSegDone_.get(), dSegIter_.get(), capIter, flags_.get(), flags_.get() + 1,
+                  reinterpret_cast<unsigned*>(flags_.get() + 8)};
     check(launch_chain_sum_shift<T>(ha, dim_, totalP, false, st_), "chain_sum_shift");
+    if (finalize && !fuseFinalize)
+        check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
+                                     flags_.get(), flags_.get() + 1, st_),
+              "finalize");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
 }
 
@@ -746,13 +762,7 @@ bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGrap
     }
     bool ok = true;
     try {
-        for (int b = 0; b < batch; ++b) {
-            enqueueIteration(false, -1);
-            if (finalize)
-                check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(),
-                                             capIter, flags_.get(), flags_.get() + 1, st_),
-                      "finalize");
-        }
+        for (int b = 0; b < batch; ++b) enqueueIteration(false, -1, nullptr, finalize, capIter);
     } catch (const HipError&) {
         ok = false;
     }
@@ -876,6 +886,8 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     check(hipMemcpyAsync(dSegDone_.get(), done.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
+    const unsigned zeroBlocks = 0;   // completion count of the fused bookkeeping
+    check(hipMemcpyAsync(flags_.get() + 8, &zeroBlocks, sizeof(unsigned), hipMemcpyHostToDevice, st_), "flags");
     autotuneScan();
     segmentedNow_ = chooseSegmented(done);
     if (segmentedNow_) {
@@ -897,12 +909,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         if (useGraph) {
             check(hipGraphLaunch(exec, st_), "hipGraphLaunch");
         } else {
-            for (int b = 0; b < batch; ++b) {
-                enqueueIteration(false, b);
-                check(launch_evolve_finalize(ns, dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
-                                             flags_.get(), flags_.get() + 1, st_),
-                      "finalize");
-            }
+            for (int b = 0; b < batch; ++b) enqueueIteration(false, b, nullptr, true, capIter);
         }
         check(hipMemcpyAsync(f, flags_.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "flags");
         check(hipStreamSynchronize(st_), "evolve");

@@ -270,7 +270,9 @@ private:
         unsigned char* moved;
         int64_t movedStride;
     };
-    void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr);
+    // finalize: also the per-segment convergence bookkeeping (iteration cap capIter)
+    void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr, bool finalize = false,
+                          int capIter = 0);
     // state of the last evolveForward
     struct Tape {
         long long iters = 0;

@@ -258,33 +258,16 @@ __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wa
     if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
 }
 
-// blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
-    __shared__ __align__(16) T tiles[256 / kWave][wave_lds_elems<T>()];
-    if (*a.allDone) return;
-    if ((int64_t)blockIdx.x < rowBlocks) {
-        chain_sum_rows<T, DIM>(a, blockIdx.x, tiles[threadIdx.x / kWave]);
-    } else {
-        const int w = threadIdx.x / kWave;
-        chain_sum_wave<T, DIM>(a, ((int64_t)blockIdx.x - rowBlocks) * (256 / kWave) + w, tiles[w]);
-    }
-}
-
-// ---------------------------------------------------------------------------------
-// 5. convergence bookkeeping (one block)
-// ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void evolve_finalize_kernel(int numSegs, int* __restrict__ segDone,
-                                                               int* __restrict__ segMoved,
-                                                               int* __restrict__ segIter, int maxIter,
-                                                               int* __restrict__ allDone,
-                                                               int* __restrict__ totalIters) {
+// Per-segment iteration count and convergence after an iteration, and the batch's all-done
+// flag, by one block of any size.
+__device__ __forceinline__ void finalize_segments(int numSegs, int* __restrict__ segDone, int* __restrict__ segMoved,
+                                                  int* __restrict__ segIter, int maxIter, int* __restrict__ allDone,
+                                                  int* __restrict__ totalIters) {
     __shared__ int active;
-    if (*allDone) return;
     if (threadIdx.x == 0) active = 0;
     __syncthreads();
     int mine = 0;
-    for (int s = threadIdx.x; s < numSegs; s += 1024) {
+    for (int s = threadIdx.x; s < numSegs; s += blockDim.x) {
         if (segDone[s]) continue;
         int it = segIter[s] + 1;
         segIter[s] = it;
@@ -298,6 +281,46 @@ __global__ __launch_bounds__(1024) void evolve_finalize_kernel(int numSegs, int*
         if (active == 0) *allDone = 1;
         *totalIters += 1;
     }
+}
+
+// blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains.  With
+// a.fin set, the last block to finish (a device-scope completion count, released by a fence
+// in every block) runs the iteration's convergence bookkeeping, saving its launch.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
+    __shared__ __align__(16) T tiles[256 / kWave][wave_lds_elems<T>()];
+    __shared__ int lastBlock;
+    if (*a.allDone) return;
+    if ((int64_t)blockIdx.x < rowBlocks) {
+        chain_sum_rows<T, DIM>(a, blockIdx.x, tiles[threadIdx.x / kWave]);
+    } else {
+        const int w = threadIdx.x / kWave;
+        chain_sum_wave<T, DIM>(a, ((int64_t)blockIdx.x - rowBlocks) * (256 / kWave) + w, tiles[w]);
+    }
+    if (!a.fin.enabled) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        lastBlock = atomicAdd(a.fin.blocksDone, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!lastBlock) return;
+    __threadfence();
+    finalize_segments(a.fin.numSegs, a.fin.segDone, a.segMoved, a.fin.segIter, a.fin.maxIter, a.fin.allDone,
+                      a.fin.totalIters);
+    if (threadIdx.x == 0) *a.fin.blocksDone = 0;
+}
+
+// ---------------------------------------------------------------------------------
+// 5. convergence bookkeeping (one block)
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void evolve_finalize_kernel(int numSegs, int* __restrict__ segDone,
+                                                               int* __restrict__ segMoved,
+                                                               int* __restrict__ segIter, int maxIter,
+                                                               int* __restrict__ allDone,
+                                                               int* __restrict__ totalIters) {
+    if (*allDone) return;
+    finalize_segments(numSegs, segDone, segMoved, segIter, maxIter, allDone, totalIters);
 }
 
 // Writes the far-away sentinel into the slots after K of every segment's last tile.

@@ -64,6 +64,18 @@ struct ShiftArgs {
     // optional: the slots the rows visit (row r: coordinate r / totalSlots, slot
     // rowSlots[r % totalSlots]); null: every slot below totalSlots
     const int* rowSlots;
+    // optional: the iteration's convergence bookkeeping (evolve_finalize) run by the last
+    // block of the sum launch; blocksDone is 0 before the launch and again after it
+    struct Finalize {
+        int enabled;
+        int numSegs;
+        int* segDone;
+        int* segIter;
+        int maxIter;
+        int* allDone;
+        int* totalIters;
+        unsigned* blocksDone;
+    } fin;
 };
 
 // L of the wave-summed chains for up to maxM sorted members: the smallest power of two
