@@ -83,3 +83,19 @@ def test_tiled_drivers_match_oracle(oracle, alg):
     with _ctx(P) as c:
         S = c.tiled(alg, 0.2, 0.4, 2500, [4, 4, 4])
     np.testing.assert_array_equal(S, oracle.tiled(alg, P, 0.2, 0.4, 2500, [4, 4, 4]))
+
+
+@pytest.mark.parametrize("alg", [1, 5, 6])
+def test_position_copy_off_matches(oracle, alg):
+    """Tiled runs read their samples from a position-ordered copy by default; reading them
+    through the position index instead (RDHIP_OPT_POSITION_COPY = 0) gives the same result."""
+    from ridgedetection_amd._lib import OPT_POSITION_COPY
+    P = synth.bezier_union(40000, dim=4, seed=12, dtype=np.float32)
+    print(f"case tiled position copy bezier4d alg {alg}", flush=True)
+    out = []
+    for copy in (1, 0):
+        with _ctx(P) as c:
+            c.set_option(OPT_POSITION_COPY, copy)
+            out.append(c.tiled(alg, 0.1, 0.2, 3000, [3, 3, 3, 3]))
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], oracle.tiled(alg, P, 0.1, 0.2, 3000, [3, 3, 3, 3]))
