@@ -7,7 +7,7 @@
 //              match (one ballot per digit bit), carries per-digit counters across its
 //              chunks in LDS, and the four waves of a block are stitched together by a
 //              per-digit prefix, so equal keys keep their input order.  For digits of at
-//              most kStageBits bits (runs of at least 8 pairs per digit and block on
+//              most kStageBits bits (runs of at least 4 pairs per digit and block on
 //              average) the block then stages its pairs in LDS in block-local sorted order
 //              and writes them out in that order: consecutive lanes write consecutive
 //              addresses inside each digit's run instead of 64 scattered runs per store.
@@ -27,8 +27,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
-constexpr int kChunks = 16;                                  // 64-key chunks per wave
-constexpr int kBlockElems = kWavesPerBlock * kChunks * kWave;  // 4096
+constexpr int kChunks = 8;                                   // 64-key chunks per wave
+constexpr int kBlockElems = kWavesPerBlock * kChunks * kWave;  // 2048
 constexpr int kMaxDigitBits = 11;
 constexpr int kScanElems = 4096;                              // entries per scan block
 
