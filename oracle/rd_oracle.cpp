@@ -45,7 +45,7 @@
 namespace oracle {
 
 // ---------------------------------------------------------------------------------
-// a1: squared Euclidean distance, accumulated from 0 in dimension order.
+// Squared Euclidean distance, accumulated from 0 in dimension order.
 //     rd/cpu/brute_force/rd_inner.hpp:41-49
 // ---------------------------------------------------------------------------------
 template <typename T>
@@ -59,7 +59,7 @@ static inline T sqdist(const T* a, const T* b, int dim) {
 }
 
 // ---------------------------------------------------------------------------------
-// a3: greedy choose.  rd/cpu/brute_force/choose.hpp:36-70 (+ rd_inner.hpp:66-86)
+// Greedy choose.  rd/cpu/brute_force/choose.hpp:36-70 (+ rd_inner.hpp:66-86)
 // Point 0 is always chosen; point i is chosen iff no already chosen point lies
 // within squared distance <= r*r.  Returns K; writes coordinates and input indices.
 // ---------------------------------------------------------------------------------
@@ -91,7 +91,7 @@ static long choose(const T* P, const long* order, long n, int dim, T r, std::vec
 }
 
 // ---------------------------------------------------------------------------------
-// a4: nearest chosen point + r-ball masked coordinate sums, for one block of samples.
+// Nearest chosen point + r-ball masked coordinate sums, for one block of samples.
 //     rd/cpu/brute_force/evolve.hpp:32-77.  Members are summed in sample order.
 //     labels (optional) receives the chosen index of every member sample, or -1.
 // ---------------------------------------------------------------------------------
@@ -152,7 +152,7 @@ static void closestSphereCenterOmp(const T* P, long n, const T* S, long K, int d
 }
 
 // ---------------------------------------------------------------------------------
-// a5: shift toward the mass centre.  rd/cpu/brute_force/evolve.hpp:129-179.
+// Shift toward the mass centre.  rd/cpu/brute_force/evolve.hpp:129-179.
 // Only the coordinates whose move is "numerically distinguishable" are updated.
 // threshold = ((2 * |mc|) * count) * eps, evaluated in T.
 // ---------------------------------------------------------------------------------
@@ -198,7 +198,7 @@ static int evolveIteration(const T* P, const std::vector<std::pair<const long*, 
     return shiftTowardMassCenter(S, sums, counts, K, dim, movedMask);
 }
 
-// a6: evolve until no coordinate moves (evolve.hpp:292-342, 344-399).  maxIter caps the
+// Evolve until no coordinate moves (evolve.hpp:292-342, 344-399).  maxIter caps the
 // number of iterations (0 = unlimited, the reference behaviour).  Returns iterations.
 template <typename T>
 static long evolve(const T* P, const std::vector<std::pair<const long*, long>>& blocks, T* S,
@@ -216,12 +216,12 @@ static long evolve(const T* P, const std::vector<std::pair<const long*, long>>& 
 }
 
 // ---------------------------------------------------------------------------------
-// a7: decimate.  rd/cpu/brute_force/decimate.hpp:40-81.  A point is removed when it has
+// Decimate.  rd/cpu/brute_force/decimate.hpp:40-81.  A point is removed when it has
 // >= 4 points (itself included) within r2, or < 3 within 2*r2 (squared: 4*r2^2).
 // Removal is immediate; the pass stops when fewer than 3 points remain; passes repeat
 // until nothing changes or at most 3 points are left.  `alive` flags are updated.
 //
-// a8: decimateNoCopy, rd/cpu/brute_force/decimate.hpp:84-127: same test, but a point is
+// decimateNoCopy, rd/cpu/brute_force/decimate.hpp:84-127: same test, but a point is
 // kept (and the pass continues) when removing it would leave fewer than 3 points.
 // ---------------------------------------------------------------------------------
 template <typename T>
@@ -264,7 +264,7 @@ static long decimateImpl(const std::vector<const T*>& pts, std::vector<unsigned 
 }
 
 // ---------------------------------------------------------------------------------
-// a9: brute-force ridge detection.  rd/cpu/brute_force/ridge_detection.inl:67-146 and
+// Brute-force ridge detection.  rd/cpu/brute_force/ridge_detection.inl:67-146 and
 // the sample-list overload :171-263 (choose over the first block only, evolve over all).
 // ---------------------------------------------------------------------------------
 template <typename T>
@@ -294,7 +294,7 @@ static long ridgeDetection(const T* P, const std::vector<std::pair<const long*, 
 }
 
 // ---------------------------------------------------------------------------------
-// a10: bounding box.  rd/utils/bounding_box.hpp:127-227, 314-418.
+// Bounding box.  rd/utils/bounding_box.hpp:127-227, 314-418.
 // ---------------------------------------------------------------------------------
 template <typename T>
 struct Box {
@@ -354,7 +354,7 @@ struct Box {
     }
 };
 
-// a11: histogram bin of a point.  rd/cpu/tiled/local_tile_tree.inl:195-276.
+// Histogram bin of a point.  rd/cpu/tiled/local_tile_tree.inl:195-276.
 // Dimension 0 varies fastest in the linear id.  An id beyond the histogram (possible
 // in the reference only through rounding, where it is undefined behaviour) is clamped.
 template <typename T>
@@ -380,7 +380,7 @@ static size_t binOf(const T* x, const std::vector<size_t>& nb, const Box<T>& bb)
 }
 
 // ---------------------------------------------------------------------------------
-// a12/a13: tile trees.
+// Tile trees.
 //   LOCAL    -- rd/cpu/tiled/local_tile_tree.inl:48-437   (TiledLocalTree)
 //   GROUPED  -- rd/cpu/tiled/grouped_tile_tree.inl:86-415, 517-624 (FIND_NEIGHBOURS)
 //   EXTENDED -- rd/cpu/tiled/grouped_tile_tree.inl:86-415, 423-510 (EXTENDED_TILE)
@@ -581,7 +581,7 @@ private:
 };
 
 // ---------------------------------------------------------------------------------
-// a14: tiled drivers.  rd/cpu/tiled/rd_tiled.hpp:756-1732; enum :102-110.
+// Tiled drivers.  rd/cpu/tiled/rd_tiled.hpp:756-1732; enum :102-110.
 // ---------------------------------------------------------------------------------
 enum TiledAlg {
     TILED_LOCAL_TREE_MIXED_RD = 0,
@@ -975,7 +975,7 @@ long treeC(int kind, const T* P, long n, int dim, T r1, long cap, const long* tp
 }  // namespace
 
 // ---------------------------------------------------------------------------------
-// a12: seeded generator.  Restates ridgedetection_amd's device generator (the counterpart
+// Seeded generator.  Restates ridgedetection_amd's device generator (the counterpart
 // of rd/gpu/device/samples_generator.cuh:27-384 with its own RNG): Philox4x32-10 (Salmon,
 // Moraes, Dror, Shaw, SC'11) with counter (sample low, sample high, stream, block) and key
 // (seed low, seed high); uniforms in (0, 1] from 24 / 53 bits; Box-Muller normals.
