@@ -30,6 +30,12 @@ __global__ void fill_int_kernel(int* __restrict__ a, int64_t n, int v, const int
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = v;
 }
 
+// out[idx[i]] = i for i < n
+__global__ void scatter_rank_kernel(const int* __restrict__ idx, int64_t n, int* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        out[idx[i]] = (int)i;
+}
+
 // a[i] = b[i] = 0 for i < n (one launch for the chain bounds of an iteration)
 __global__ void zero_pair_kernel(int* __restrict__ a, int* __restrict__ b, int64_t n, const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -327,12 +333,26 @@ void Engine<T>::prepareRows(const std::vector<int>& done) {
     for (size_t s = 0; s < segs_.size(); ++s)
         if (!done[s])
             for (int k = 0; k < K[s]; ++k) rows.push_back((int)(segs_[s].sOff + k));
-    if (rows.empty() || 4 * (int64_t)rows.size() > std::max<int64_t>(keyBound_, 1)) return;
+    denseKeys_ = false;
+    if (rows.empty()) return;
+    // a single cloud keeps its slot keys (already dense); segment sets always list
+    if (!compact() && 4 * (int64_t)rows.size() > std::max<int64_t>(keyBound_, 1)) return;
     rowSlots_.reserve(rows.size());
     check(hipMemcpyAsync(rowSlots_.get(), rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, st_),
           "row slots");
     numRowSlots_ = (int64_t)rows.size();
     rowSlotsVersion_ = version_;
+    if (compact()) {
+        // sort keys of a batch: a slot's index in the list (the active clouds' live chosen
+        // points only, so the sort spans fewer key bits as clouds converge)
+        const int64_t kb = std::max<int64_t>(keyBound_, 1);
+        denseOf_.reserve(kb);
+        hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, denseOf_.get(), kb, -1, nullptr);
+        hipLaunchKernelGGL(scatter_rank_kernel, dim3(grid_for(numRowSlots_)), dim3(256), 0, st_, rowSlots_.get(),
+                           numRowSlots_, denseOf_.get());
+        check(hipGetLastError(), "dense keys");
+        denseKeys_ = true;
+    }
 }
 
 template <typename T>
@@ -541,10 +561,12 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
         return;
     }
     int* labels = log ? log->labels : labels_.get();
+    const bool rowsValid = rowSlotsVersion_ == version_ && numRowSlots_ > 0;
+    const bool dense = rowsValid && denseKeys_ && compact() && !log && !record;
     ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                    dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
                    compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
-                   compact() ? flags_.get() + 4 : nullptr};
+                   compact() ? flags_.get() + 4 : nullptr, dense ? denseOf_.get() : nullptr};
     // one block per work item (batches: the active-item list, or persistent blocks on its
     // queue when that form is selected)
     check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads,
@@ -554,8 +576,9 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
     // positions the sort and the member passes cover: the dense active range of a batch
     // (it only shrinks; graphs are captured again when it does), else every position
     const int64_t totalP = compact() ? std::max<int64_t>(activeWork_, 1) : totalP_;
-    // labels are live slots < keyBound_ (counts only shrink between refreshes)
-    const int64_t kb = std::max<int64_t>(keyBound_, 1);
+    // labels are live slots < keyBound_ (counts only shrink between refreshes), or in a
+    // batch their indices in the live-slot list
+    const int64_t kb = dense ? numRowSlots_ : std::max<int64_t>(keyBound_, 1);
     // members sorted by (label, position); the last pass writes their coordinates in
     // chain order straight into sortedP_
     const SortGather<T> sg{es.P, es.stride, es.index, sortedP_.get(), totalP_, dim_};
@@ -583,10 +606,11 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
                     log ? log->moved : (record ? moved_.get() : nullptr),
                     log ? log->movedStride : totalSlots_, longList_.get(), longShift, nullptr};
-    if (rowSlotsVersion_ == version_ && numRowSlots_ > 0) {
+    if (rowsValid) {
         // rows over the live slots of the evolve call only
         ha.totalSlots = numRowSlots_;
         ha.rowSlots = rowSlots_.get();
+        ha.denseChains = dense ? 1 : 0;
     }
     // A single cloud: the sum launch's last block does the convergence bookkeeping (one
     // launch fewer).  Segment sets keep the separate launch: the device-scope fence every
@@ -928,6 +952,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
                 if (!doneNow[it.seg]) remaining += it.end - it.start;
             if (2 * remaining <= activeWork_) {
                 rebuildActive(doneNow);
+                prepareRows(doneNow);
                 if (exec) {
                     (void)hipGraphExecDestroy(exec);
                     exec = nullptr;

@@ -211,6 +211,8 @@ private:
     DevBuf<int> rowSlots_;
     int64_t numRowSlots_ = 0;
     uint64_t rowSlotsVersion_ = ~uint64_t(0);
+    DevBuf<int> denseOf_;     // batch sort key of every live slot (-1: none)
+    bool denseKeys_ = false;
     void prepareRows(const std::vector<int>& done);
     EvolveSource evolveSource() const {
         if (useIndex_ && psegValid_ && options.positionCopy) return {Pseg_.get(), psegStride_, nullptr};

@@ -215,12 +215,13 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
     const int d = (int)(row / max<int64_t>(a.totalSlots, 1));
     const int64_t rs = row - (int64_t)d * a.totalSlots;
     const int64_t slot = (a.rowSlots && d < DIM) ? (int64_t)a.rowSlots[rs] : rs;
+    const int64_t chain = a.denseChains ? rs : slot;
     int seg = -1, off = 0, cnt = 0;
     if (d < DIM) {
         seg = a.slotSeg[slot];
         if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
-            off = a.chainOff[slot];
-            cnt = a.chainEnd[slot] - off;
+            off = a.chainOff[chain];
+            cnt = a.chainEnd[chain] - off;
             // a chain holding a multiple of L is summed by a wave (chain_sum_wave)
             const int64_t s0 = off, e1 = (int64_t)off + cnt - 1;
             if (a.longList && cnt > 0 && (e1 >> a.longShift) >= ((s0 + (int64_t(1) << a.longShift) - 1) >> a.longShift)) {
@@ -248,12 +249,13 @@ __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wa
     const int64_t e = wave / DIM;
     const int d = (int)(wave - e * DIM);
     if ((e << a.longShift) >= *a.mPtr) return;
-    const int slot = a.longList[e];
-    if (slot < 0) return;
+    const int key = a.longList[e];
+    if (key < 0) return;
+    const int slot = a.denseChains ? a.rowSlots[key] : key;
     const int seg = a.slotSeg[slot];
     if (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg]) return;
-    const int off = a.chainOff[slot];
-    const int cnt = a.chainEnd[slot] - off;
+    const int off = a.chainOff[key];
+    const int cnt = a.chainEnd[key] - off;
     const T sum = group_ordered_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt, cnt, tile);
     if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
 }

@@ -30,6 +30,9 @@ struct ScanArgs {
     int* outKeys;
     int* outPos;
     const int* numItemsDev;
+    // optional (compact form): the key written for slot s is denseOf[s], its index among
+    // the live slots of the evolve call, so that the member sort spans only those
+    const int* denseOf;
 };
 
 template <typename T>
@@ -76,6 +79,9 @@ struct ShiftArgs {
         int* totalIters;
         unsigned* blocksDone;
     } fin;
+    // chains are indexed by the dense key (row r's chain is r, a listed chain's slot is
+    // rowSlots[key]) instead of the slot
+    int denseChains;
 };
 
 // L of the wave-summed chains for up to maxM sorted members: the smallest power of two

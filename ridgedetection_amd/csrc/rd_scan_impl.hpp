@@ -21,7 +21,7 @@ __device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem i
     int* __restrict__ cp = a.outKeys ? a.outPos + it.pad - it.start : nullptr;
     auto put = [&](int i, int label) {
         if (ck) {
-            ck[i] = label;
+            ck[i] = (label >= 0 && a.denseOf) ? a.denseOf[label] : label;
             cp[i] = (int)(sd.pOff + i);
         } else {
             lab[i] = label;
