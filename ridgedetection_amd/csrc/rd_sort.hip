@@ -11,9 +11,8 @@
 //              average) the block then stages its pairs in LDS in block-local sorted order
 //              and writes them out in that order: consecutive lanes write consecutive
 //              addresses inside each digit's run instead of 64 scattered runs per store.
-//              Wider digits leave runs too short to gain, and the larger LDS footprint
-//              halves the blocks per CU, which costs more than it saves in launches of
-//              more than kStageMaxBlocks blocks: those write straight from the ranks.
+//              Wider digits leave runs too short to gain: they write straight from the
+//              ranks.
 // Negative keys are dropped by the first pass, which also writes the count of kept
 // pairs.  Used to list the members of every chosen point in point order (evolve) and to
 // bin points by tile (tree build).
@@ -145,7 +144,6 @@ struct GatherSink {
 };
 
 constexpr int kStageBits = 9;
-constexpr int kStageMaxBlocks = 4096;   // larger launches keep the smaller LDS footprint
 
 template <bool kStage, typename Sink>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
@@ -306,7 +304,7 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 template <typename Sink>
 void launch_scatter(const int* inK, const int* inV, const int64_t* n, int shift, int w, int blocks, const int* ghist,
                     int* oK, const Sink& sink, const int* skip, hipStream_t stream) {
-    if (w <= kStageBits && blocks <= kStageMaxBlocks)
+    if (w <= kStageBits)
         hipLaunchKernelGGL((radix_scatter_kernel<true, Sink>), dim3((unsigned)blocks), dim3(kThreads), 0, stream, inK,
                            inV, n, shift, w, blocks, ghist, oK, sink, skip);
     else
