@@ -73,32 +73,44 @@ __global__ __launch_bounds__(256) void dec_elems_kernel(DecArgs a) {
 }
 
 // Bounding box of every chunk of kDecChunk consecutive elements: lo at [c][0, DIM), hi at
-// [c][DIM, 2 DIM).  One wave per chunk.
-constexpr int kDecChunk = 64;
+// [c][DIM, 2 DIM).  kBoxLanes lanes per chunk, each folding every kBoxLanes-th element.
+constexpr int kDecChunk = 16;
+constexpr int kBoxLanes = kDecChunk < kWave ? kDecChunk : kWave;
+constexpr int kBoxChunksPerBlock = 256 / kBoxLanes;
+static_assert((kDecChunk & (kDecChunk - 1)) == 0 && kDecChunk >= 8, "kDecChunk: a power of two");
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* __restrict__ S, int64_t ss,
                                                             T* __restrict__ boxes, int64_t maxChunks) {
-    const int64_t c = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
-    if (c >= maxChunks) return;
+    const int64_t c = (int64_t)blockIdx.x * kBoxChunksPerBlock + threadIdx.x / kBoxLanes;
+    if (c >= maxChunks) return;   // whole chunk groups leave together: the shuffles stay within one
     const int64_t total = *a.totalElems;
-    const int64_t e = c * kDecChunk + lane_id();
+    const int sub = threadIdx.x % kBoxLanes;
     T lo[DIM], hi[DIM];
-    const int se = e < total ? a.elemSlot[e] : 0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const T x = e < total ? S[d * ss + se] : T(0);
-        lo[d] = e < total ? x : FpTraits<T>::big;
-        hi[d] = e < total ? x : -FpTraits<T>::big;
+        lo[d] = FpTraits<T>::big;
+        hi[d] = -FpTraits<T>::big;
+    }
+    for (int k = sub; k < kDecChunk; k += kBoxLanes) {
+        const int64_t e = c * kDecChunk + k;
+        if (e >= total) break;
+        const int se = a.elemSlot[e];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const T x = S[d * ss + se];
+            lo[d] = min(lo[d], x);
+            hi[d] = max(hi[d], x);
+        }
     }
 #pragma unroll
-    for (int m = 1; m < kWave; m <<= 1)
+    for (int m = 1; m < kBoxLanes; m <<= 1)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             lo[d] = min(lo[d], __shfl_xor(lo[d], m));
             hi[d] = max(hi[d], __shfl_xor(hi[d], m));
         }
-    if (lane_id() == 0)
+    if (sub == 0)
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             boxes[c * 2 * DIM + d] = lo[d];
@@ -527,7 +539,8 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     const unsigned blocks = (unsigned)((maxElems + 1 + 255) / 256);
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
-        hipLaunchKernelGGL((dec_chunk_box_kernel<T, DIM>), dim3((unsigned)((maxChunks + 3) / 4)), dim3(256), 0, st, a,
+        hipLaunchKernelGGL((dec_chunk_box_kernel<T, DIM>),
+                           dim3((unsigned)((maxChunks + kBoxChunksPerBlock - 1) / kBoxChunksPerBlock)), dim3(256), 0, st, a,
                            (const T*)S, sStride, boxes, maxChunks);
         hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, S, sStride, groupRSqr,
                            0, maxElems, (const T*)boxes);
