@@ -7,6 +7,8 @@
 //                  (parent own, parent neighbours) that is near the cell, counted, emitted
 //                  and stably sorted by child id;
 //   3. bounds   -- LOCAL only: tight bounds of every child's own samples;
+// after the last level, GROUPED leaves take their neighbour lists from a device bit matrix
+// of every pair's closed-box test;
 // and the host keeps the skeleton (counts, bounds, leaf/split decisions).  Cell bounds are
 // computed on the host with the reference's expressions (min + c * step, last cell ending at
 // the parent max) and uploaded as per-dimension interval tables, so device and host agree
@@ -210,41 +212,6 @@ __global__ __launch_bounds__(256) void nearby_kernel(const T* __restrict__ P, in
     }
 }
 
-// tight bounds of segment s = idx[off[s] .. end[s]) (min/max with the reference's comparisons)
-template <typename T>
-__global__ __launch_bounds__(256) void seg_bounds_kernel(const T* __restrict__ P, int64_t ps, int dim,
-                                                         const int* __restrict__ idx,
-                                                         const int* __restrict__ off, const int* __restrict__ end,
-                                                         const int* __restrict__ sel, T* __restrict__ out) {
-    __shared__ T smn[256], smx[256];
-    const int s = sel[blockIdx.x];
-    const int o = off[s], e = end[s];
-    for (int d = 0; d < dim; ++d) {
-        T mn = FpTraits<T>::big, mx = -FpTraits<T>::big;
-        for (int i = o + threadIdx.x; i < e; i += 256) {
-            const T v = P[d * ps + idx[i]];
-            mn = (v < mn) ? v : mn;
-            mx = (v > mx) ? v : mx;
-        }
-        smn[threadIdx.x] = mn;
-        smx[threadIdx.x] = mx;
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if (threadIdx.x < h) {
-                const T a = smn[threadIdx.x + h], b = smx[threadIdx.x + h];
-                smn[threadIdx.x] = (a < smn[threadIdx.x]) ? a : smn[threadIdx.x];
-                smx[threadIdx.x] = (b > smx[threadIdx.x]) ? b : smx[threadIdx.x];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            out[(int64_t)blockIdx.x * 2 * dim + 2 * d] = smn[0];
-            out[(int64_t)blockIdx.x * 2 * dim + 2 * d + 1] = smx[0];
-        }
-        __syncthreads();
-    }
-}
-
 // Tight bounds of the whole cloud P[:, 0..n): per-block partial min/max (order-free, so
 // exact), then one block combines the partials.
 template <typename T>
@@ -263,6 +230,71 @@ __device__ __forceinline__ void block_minmax(T& mn, T& mx, T* smn, T* smx) {
     mn = smn[0];
     mx = smx[0];
     __syncthreads();
+}
+
+// Tight bounds of the kept children of a level, one block per item of up to kBoundItem
+// sorted positions: per-item partial min / max with the reference's comparisons.  Min and max
+// are order-free, so folding the partials equals one pass over the child, and a large child
+// is reduced by many blocks.
+constexpr int kBoundItem = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(256) void item_bounds_kernel(const T* __restrict__ P, int64_t ps, int dim,
+                                                          const int* __restrict__ idx,
+                                                          const WorkItem* __restrict__ items, T* __restrict__ partial) {
+    __shared__ T smn[256], smx[256];
+    const WorkItem it = items[blockIdx.x];
+    T mn[kMaxDim], mx[kMaxDim];
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) {
+        mn[d] = FpTraits<T>::big;
+        mx[d] = -FpTraits<T>::big;
+    }
+    for (int i = it.start + threadIdx.x; i < it.end; i += 256) {
+        const int p = idx[i];
+#pragma unroll
+        for (int d = 0; d < kMaxDim; ++d) {
+            if (d >= dim) break;
+            const T v = P[d * ps + p];
+            mn[d] = (v < mn[d]) ? v : mn[d];
+            mx[d] = (v > mx[d]) ? v : mx[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) {
+        if (d >= dim) break;   // dim is uniform over the block: every thread meets the same barriers
+        block_minmax(mn[d], mx[d], smn, smx);
+        if (threadIdx.x == 0) {
+            partial[(int64_t)blockIdx.x * 2 * dim + 2 * d] = mn[d];
+            partial[(int64_t)blockIdx.x * 2 * dim + 2 * d + 1] = mx[d];
+        }
+    }
+}
+
+// GROUPED neighbours: the closed boxes of leaves l and j overlap when, in every dimension,
+// an endpoint of one lies within the other's [mn, mx].  Bit j of row l is that test for
+// j != l (rows row0 .. row0 + rows, `words` 64-bit words a row).  A wave evaluates the 64
+// candidates of one word and writes its ballot; words * 64 threads a row keep every wave
+// inside one row.
+template <typename T>
+__global__ __launch_bounds__(256) void leaf_overlap_kernel(const T* __restrict__ mn, const T* __restrict__ mx, int L,
+                                                           int dim, int row0, int rows, int words,
+                                                           unsigned long long* __restrict__ bits) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t perRow = (int64_t)words * 64;
+    const int64_t r = t / perRow;
+    if (r >= rows) return;
+    const int j = (int)(t - r * perRow);
+    const int l = row0 + (int)r;
+    bool hit = j < L && j != l;
+    for (int d = 0; d < dim && hit; ++d) {
+        const T amn = mn[(int64_t)l * dim + d], amx = mx[(int64_t)l * dim + d];
+        const T bmn = mn[(int64_t)j * dim + d], bmx = mx[(int64_t)j * dim + d];
+        hit = (bmn >= amn && bmn <= amx) || (bmx >= amn && bmx <= amx) || (amn >= bmn && amn <= bmx) ||
+              (amx >= bmn && amx <= bmx);
+    }
+    const unsigned long long w = __ballot(hit);
+    if (lane_id() == 0) bits[r * words + j / 64] = w;
 }
 
 template <typename T>
@@ -333,17 +365,6 @@ size_t count_spheres(const std::vector<T>& dist, T r) {
     return cnt;
 }
 
-template <typename T>
-bool boxes_overlap(const std::vector<T>& amn, const std::vector<T>& amx, const std::vector<T>& bmn,
-                   const std::vector<T>& bmx) {
-    for (size_t d = 0; d < amn.size(); ++d) {
-        bool res = (bmn[d] >= amn[d] && bmn[d] <= amx[d]) || (bmx[d] >= amn[d] && bmx[d] <= amx[d]) ||
-                   (amn[d] >= bmn[d] && amn[d] <= bmx[d]) || (amx[d] >= bmn[d] && amx[d] <= bmx[d]);
-        if (!res) return false;
-    }
-    return true;
-}
-
 }  // namespace
 
 template <typename T>
@@ -412,6 +433,14 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         int z = 0;
         check(hipMemcpyAsync(zeroFlag.get(), &z, sizeof(int), hipMemcpyHostToDevice, st_), "flag");
     }
+    // Level scratch, reused by every level (each level's sorted own lists stay with its tiles
+    // in bufs_).
+    DevBuf<ParentDev<T>> dPar;
+    DevBuf<WorkItem> dOwnItems, dNearItems, dBoundItems;
+    DevBuf<T> dLo, dHi, dPartial;
+    DevBuf<int> keys, vals, sortedKeys, cOff, cEnd, counts, partials, pk, pv, sk;
+    DevBuf<int64_t> dN, dM, pairTotal, pn, pm;
+    DevBuf<unsigned char> ws, ws2;
     while (!parents.empty()) {
         if (levels_ > 64) throw std::runtime_error("tile tree exceeds 64 levels (identical points?)");
         // ---- parent descriptors and cell interval tables
@@ -476,9 +505,6 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         }
         if (ownTotal >= (int64_t(1) << 31) || posTotal >= (int64_t(1) << 31))
             throw std::runtime_error("tile level too large");
-        DevBuf<ParentDev<T>> dPar;
-        DevBuf<WorkItem> dOwnItems, dNearItems;
-        DevBuf<T> dLo, dHi;
         dPar.reserve(pdev.size());
         check(hipMemcpyAsync(dPar.get(), pdev.data(), sizeof(ParentDev<T>) * pdev.size(), hipMemcpyHostToDevice, st_),
               "parents");
@@ -495,7 +521,6 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         }
 
         // ---- own samples: bin + stable sort by child
-        DevBuf<int> keys, vals;
         keys.reserve(std::max<int64_t>(ownTotal, 1));
         vals.reserve(std::max<int64_t>(ownTotal, 1));
         if (!ownItems.empty())
@@ -506,19 +531,15 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         const int levelOwnBuf = (int)bufs_.size();
         bufs_.emplace_back(new DevBuf<int>());
         int* sortedVals = bufs_.back()->reserve(std::max<int64_t>(ownTotal, 1));
-        DevBuf<int> sortedKeys;
         sortedKeys.reserve(std::max<int64_t>(ownTotal, 1));
-        DevBuf<int64_t> dN, dM;
         dN.reserve(1);
         dM.reserve(1);
         check(hipMemcpyAsync(dN.get(), &ownTotal, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
-        DevBuf<unsigned char> ws;
         ws.reserve(sort_workspace_bytes(std::max<int64_t>(ownTotal, 1), std::max(totalChildren, 2)));
         if (ownTotal > 0)
             check(stable_sort_by_key(keys.get(), vals.get(), dN.get(), ownTotal, std::max(totalChildren, 2),
                                      sortedKeys.get(), sortedVals, dM.get(), ws.get(), st_),
                   "sort own");
-        DevBuf<int> cOff, cEnd;
         cOff.reserve(totalChildren);
         cEnd.reserve(totalChildren);
         hipLaunchKernelGGL(zero_kernel, dim3(grid_of(totalChildren)), dim3(256), 0, st_, cOff.get(),
@@ -536,7 +557,6 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         std::vector<int> nOff(totalChildren, 0), nEnd(totalChildren, 0);
         int levelNbBuf = -1;
         if (ext && posTotal > 0) {
-            DevBuf<int> counts;
             counts.reserve(posTotal + 1);
             dNearItems.reserve(nearItems.size());
             check(hipMemcpyAsync(dNearItems.get(), nearItems.data(), sizeof(WorkItem) * nearItems.size(),
@@ -546,9 +566,7 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
             hipLaunchKernelGGL((nearby_kernel<T, true>), dim3((unsigned)nearItems.size()), dim3(256), 0, st_, P, ps,
                                dim, r1, dPar.get(), dNearItems.get(), dLo.get(), dHi.get(), counts.get(), nullptr,
                                nullptr);
-            DevBuf<int> partials;
             partials.reserve(scan_partials_count(posTotal + 1));
-            DevBuf<int64_t> pairTotal;
             pairTotal.reserve(1);
             check(exclusive_scan_inplace(counts.get(), posTotal + 1, partials.get(), pairTotal.get(), st_), "scan");
             int64_t pairs = 0;
@@ -559,18 +577,15 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
             bufs_.emplace_back(new DevBuf<int>());
             int* nbVals = bufs_.back()->reserve(std::max<int64_t>(pairs, 1));
             if (pairs > 0) {
-                DevBuf<int> pk, pv, sk;
                 pk.reserve(pairs);
                 pv.reserve(pairs);
                 sk.reserve(pairs);
                 hipLaunchKernelGGL((nearby_kernel<T, false>), dim3((unsigned)nearItems.size()), dim3(256), 0, st_, P,
                                    ps, dim, r1, dPar.get(), dNearItems.get(), dLo.get(), dHi.get(), counts.get(),
                                    pk.get(), pv.get());
-                DevBuf<int64_t> pn, pm;
                 pn.reserve(1);
                 pm.reserve(1);
                 check(hipMemcpyAsync(pn.get(), &pairs, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
-                DevBuf<unsigned char> ws2;
                 ws2.reserve(sort_workspace_bytes(pairs, std::max(totalChildren, 2)));
                 check(stable_sort_by_key(pk.get(), pv.get(), pn.get(), pairs, std::max(totalChildren, 2), sk.get(), nbVals,
                                          pm.get(), ws2.get(), st_),
@@ -595,19 +610,36 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
             if (hEnd[c] - hOff[c] >= 2) keep.push_back(c);
         std::vector<T> tb;
         if (local && !keep.empty()) {
-            DevBuf<int> dSel;
-            dSel.reserve(keep.size());
-            check(hipMemcpyAsync(dSel.get(), keep.data(), sizeof(int) * keep.size(), hipMemcpyHostToDevice, st_), "sel");
-            DevBuf<T> out;
-            out.reserve(keep.size() * 2 * dim);
-            // cOff/cEnd were overwritten by the neighbour runs only for EXTENDED trees
-            check(hipMemcpyAsync(cOff.get(), hOff.data(), sizeof(int) * totalChildren, hipMemcpyHostToDevice, st_), "o");
-            check(hipMemcpyAsync(cEnd.get(), hEnd.data(), sizeof(int) * totalChildren, hipMemcpyHostToDevice, st_), "o");
-            hipLaunchKernelGGL((seg_bounds_kernel<T>), dim3((unsigned)keep.size()), dim3(256), 0, st_, P, ps, dim,
-                               sortedVals, cOff.get(), cEnd.get(), dSel.get(), out.get());
-            tb.resize(keep.size() * 2 * dim);
-            check(hipMemcpyAsync(tb.data(), out.get(), sizeof(T) * tb.size(), hipMemcpyDeviceToHost, st_), "bounds");
+            std::vector<WorkItem> bi;
+            for (size_t k = 0; k < keep.size(); ++k)
+                for (int s = hOff[keep[k]]; s < hEnd[keep[k]]; s += kBoundItem)
+                    bi.push_back({(int)k, s, std::min(hEnd[keep[k]], s + kBoundItem), 0});
+            dBoundItems.reserve(bi.size());
+            check(hipMemcpyAsync(dBoundItems.get(), bi.data(), sizeof(WorkItem) * bi.size(), hipMemcpyHostToDevice,
+                                 st_),
+                  "items");
+            dPartial.reserve(bi.size() * 2 * dim);
+            hipLaunchKernelGGL((item_bounds_kernel<T>), dim3((unsigned)bi.size()), dim3(256), 0, st_, P, ps, dim,
+                               sortedVals, dBoundItems.get(), dPartial.get());
+            check(hipGetLastError(), "bounds");
+            std::vector<T> part(bi.size() * 2 * dim);
+            check(hipMemcpyAsync(part.data(), dPartial.get(), sizeof(T) * part.size(), hipMemcpyDeviceToHost, st_),
+                  "bounds");
             check(hipStreamSynchronize(st_), "bounds");
+            tb.resize(keep.size() * 2 * dim);
+            for (size_t k = 0; k < keep.size(); ++k)
+                for (int d = 0; d < dim; ++d) {
+                    tb[k * 2 * dim + 2 * d] = FpTraits<T>::big;
+                    tb[k * 2 * dim + 2 * d + 1] = -FpTraits<T>::big;
+                }
+            for (size_t b = 0; b < bi.size(); ++b) {
+                T* t = &tb[(size_t)bi[b].seg * 2 * dim];
+                const T* q = &part[b * 2 * dim];
+                for (int d = 0; d < dim; ++d) {
+                    t[2 * d] = (q[2 * d] < t[2 * d]) ? q[2 * d] : t[2 * d];
+                    t[2 * d + 1] = (q[2 * d + 1] > t[2 * d + 1]) ? q[2 * d + 1] : t[2 * d + 1];
+                }
+            }
         }
 
         // ---- children (in each parent's linear order), leaf / split decisions
@@ -681,13 +713,44 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         li.sphereCap = count_spheres(t.dist, r1);
         li.nbCnt = ext ? t.nbCnt : 0;
     }
-    if (kind == TREE_GROUPED) {
+    if (kind == TREE_GROUPED && L > 0) {
+        // every pair tested on the device, rows of the bit matrix decoded in increasing j
+        std::vector<T> hmn((size_t)L * dim), hmx((size_t)L * dim);
         for (int l = 0; l < L; ++l)
-            for (int j = 0; j < L; ++j)
-                if (j != l && boxes_overlap(leaves_[l].mn, leaves_[l].mx, leaves_[j].mn, leaves_[j].mx)) {
-                    leaves_[l].nbLeaves.push_back(j);
-                    leaves_[l].nbCnt += leaves_[j].ownCnt;
-                }
+            for (int d = 0; d < dim; ++d) {
+                hmn[(size_t)l * dim + d] = leaves_[l].mn[d];
+                hmx[(size_t)l * dim + d] = leaves_[l].mx[d];
+            }
+        DevBuf<T> dmn, dmx;
+        dmn.reserve(hmn.size());
+        dmx.reserve(hmx.size());
+        check(hipMemcpyAsync(dmn.get(), hmn.data(), sizeof(T) * hmn.size(), hipMemcpyHostToDevice, st_), "boxes");
+        check(hipMemcpyAsync(dmx.get(), hmx.data(), sizeof(T) * hmx.size(), hipMemcpyHostToDevice, st_), "boxes");
+        const int words = (L + 63) / 64;
+        const int rowsPer = (int)std::max<int64_t>(1, std::min<int64_t>(L, (int64_t(8) << 20) / words));
+        DevBuf<unsigned long long> dbits;
+        dbits.reserve((size_t)rowsPer * words);
+        std::vector<unsigned long long> hbits((size_t)rowsPer * words);
+        for (int row0 = 0; row0 < L; row0 += rowsPer) {
+            const int rows = std::min(rowsPer, L - row0);
+            const int64_t threads = (int64_t)rows * words * 64;
+            hipLaunchKernelGGL((leaf_overlap_kernel<T>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st_,
+                               dmn.get(), dmx.get(), L, dim, row0, rows, words, dbits.get());
+            check(hipGetLastError(), "overlap");
+            check(hipMemcpyAsync(hbits.data(), dbits.get(), sizeof(unsigned long long) * rows * words,
+                                 hipMemcpyDeviceToHost, st_),
+                  "overlap");
+            check(hipStreamSynchronize(st_), "overlap");
+            for (int r = 0; r < rows; ++r) {
+                LeafInfo<T>& li = leaves_[row0 + r];
+                for (int w = 0; w < words; ++w)
+                    for (unsigned long long m = hbits[(size_t)r * words + w]; m; m &= m - 1) {
+                        const int j = w * 64 + __builtin_ctzll(m);
+                        li.nbLeaves.push_back(j);
+                        li.nbCnt += leaves_[j].ownCnt;
+                    }
+            }
+        }
     }
     for (auto& li : leaves_) totalPositions_ += li.ownCnt + li.nbCnt;
     if (totalPositions_ >= (int64_t(1) << 31)) throw std::runtime_error("tiled layout exceeds 2^31 samples");
