@@ -73,7 +73,8 @@ __global__ __launch_bounds__(256) void dec_elems_kernel(DecArgs a) {
 }
 
 // Bounding box of every chunk of kDecChunk consecutive elements: lo at [c][0, DIM), hi at
-// [c][DIM, 2 DIM).  kBoxLanes lanes per chunk, each folding every kBoxLanes-th element.
+// [c][DIM, 2 DIM).  kBoxLanes lanes per chunk, each folding every kBoxLanes-th element.  The
+// kernel also stages the coordinates in element order (E[e][0, DIM)) for the count.
 constexpr int kDecChunk = 16;
 constexpr int kBoxLanes = kDecChunk < kWave ? kDecChunk : kWave;
 constexpr int kBoxChunksPerBlock = 256 / kBoxLanes;
@@ -81,7 +82,8 @@ static_assert((kDecChunk & (kDecChunk - 1)) == 0 && kDecChunk >= 8, "kDecChunk: 
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* __restrict__ S, int64_t ss,
-                                                            T* __restrict__ boxes, int64_t maxChunks) {
+                                                            T* __restrict__ boxes, int64_t maxChunks,
+                                                            T* __restrict__ E) {
     const int64_t c = (int64_t)blockIdx.x * kBoxChunksPerBlock + threadIdx.x / kBoxLanes;
     if (c >= maxChunks) return;   // whole chunk groups leave together: the shuffles stay within one
     const int64_t total = *a.totalElems;
@@ -99,6 +101,7 @@ __global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             const T x = S[d * ss + se];
+            E[e * DIM + d] = x;
             lo[d] = min(lo[d], x);
             hi[d] = max(hi[d], x);
         }
@@ -118,51 +121,115 @@ __global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* 
         }
 }
 
+// Boxes of kDecSuper consecutive chunks (the hull of their boxes), after the chunk boxes.
+constexpr int kDecSuper = 16;
+
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __restrict__ S, int64_t ss,
+__global__ __launch_bounds__(256) void dec_super_box_kernel(T* __restrict__ boxes, int64_t maxChunks,
+                                                            int64_t maxSupers) {
+    const int64_t sc = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (sc >= maxSupers) return;
+    T lo[DIM], hi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        lo[d] = FpTraits<T>::big;
+        hi[d] = -FpTraits<T>::big;
+    }
+    for (int64_t c = sc * kDecSuper, c1 = min(maxChunks, (sc + 1) * kDecSuper); c < c1; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            lo[d] = min(lo[d], boxes[c * 2 * DIM + d]);
+            hi[d] = max(hi[d], boxes[c * 2 * DIM + DIM + d]);
+        }
+    T* out = boxes + (maxChunks + sc) * 2 * DIM;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        out[d] = lo[d];
+        out[DIM + d] = hi[d];
+    }
+}
+
+// Neighbour counts within r2 and 2 r2 of every element in its group (fill = 0), then the
+// CSR rows (fill = 1).  A wave walks the union of its lanes' chunk ranges with wave-uniform
+// indices, so candidate coordinates and boxes are broadcast loads; each lane tests only the
+// candidates of its own group, in increasing element order, and only in chunks whose box
+// passes its own test, as a walk of its own would.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __restrict__ E,
                                                         const T* __restrict__ groupRSqr, int fill,
-                                                        int64_t maxElems, const T* __restrict__ boxes) {
+                                                        int64_t maxElems, const T* __restrict__ boxes,
+                                                        int64_t maxChunks) {
     const int64_t total = *a.totalElems;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= total) {
-        if (!fill && e <= maxElems) a.rowOff[e] = 0;   // tail of the offsets scan
-        return;
-    }
-    const int g = a.elemGroup[e];
-    const int o = a.groupOff[g], n = a.groupCnt[g];
-    const T rSqr = groupRSqr[g];
-    const T r4 = T(4) * rSqr;
-    T p[DIM];
-    const int se = a.elemSlot[e];
+    const bool valid = e < total;
+    if (!valid && !fill && e <= maxElems) a.rowOff[e] = 0;   // tail of the offsets scan
+    if (__ballot(valid) == 0) return;                        // the whole wave lies past the end
+    int o = 0, n = 0;
+    T rSqr = T(0), p[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) p[d] = S[d * ss + se];
+    for (int d = 0; d < DIM; ++d) p[d] = T(0);
+    if (valid) {
+        const int g = a.elemGroup[e];
+        o = a.groupOff[g];
+        n = a.groupCnt[g];
+        rSqr = groupRSqr[g];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) p[d] = E[e * DIM + d];
+    }
+    const T r4 = T(4) * rSqr;
     int c1 = 0, c2 = 0;
-    int w = fill ? a.rowOff[e] : 0;
+    int w = fill && valid ? a.rowOff[e] : 0;
     // Chunks whose box lies farther than 2 r2 hold no neighbour: the squared distance to a
-    // box bounds every member's from below; the margin covers the rounding of both sums.
+    // box bounds every member's from below; the margin covers the rounding of both sums.  A
+    // super box holds its chunks' boxes, and every step of the box distance is monotone, so
+    // a skipped super box skips only chunks that would be skipped one by one.
     const T skipAbove = r4 * (T(1) + T(16 * (DIM + 2)) * FpTraits<T>::unit_roundoff) + FpTraits<T>::tiny;
-    for (int c = o / kDecChunk, cEnd = n > 0 ? (o + n - 1) / kDecChunk : c - 1; c <= cEnd; ++c) {
+    auto boxDist = [&](const T* b) {
         T bd = T(0);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const T t = max(max(boxes[c * 2 * DIM + d] - p[d], p[d] - boxes[c * 2 * DIM + DIM + d]), T(0));
+            const T t = max(max(b[d] - p[d], p[d] - b[DIM + d]), T(0));
             bd = bd + t * t;
         }
-        if (bd > skipAbove) continue;
-        for (int j = max(o, c * kDecChunk), j1 = min(o + n, (c + 1) * kDecChunk); j < j1; ++j) {
-            const int sj = a.elemSlot[j];
-            T q[DIM];
+        return bd;
+    };
+    const bool any = valid && n > 0;
+    const int cFirst = any ? o / kDecChunk : INT_MAX, cLast = any ? (o + n - 1) / kDecChunk : -1;
+    int wFirst = cFirst, wLast = cLast;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) q[d] = S[d * ss + sj];
-            const T dd = exact_sqdist<T, DIM>(p, q);
-            const bool in1 = dd <= rSqr;
-            const bool in2 = dd <= r4;
-            c1 += in1;
-            c2 += in2;
-            if (fill && in2 && j != e) a.nbr[w++] = (j << 1) | (in1 ? 1 : 0);
+    for (int m = 1; m < kWave; m <<= 1) {
+        wFirst = min(wFirst, __shfl_xor(wFirst, m));
+        wLast = max(wLast, __shfl_xor(wLast, m));
+    }
+    wFirst = __builtin_amdgcn_readfirstlane(wFirst);
+    wLast = __builtin_amdgcn_readfirstlane(wLast);
+    for (int sc = wFirst / kDecSuper; wFirst <= wLast && sc <= wLast / kDecSuper; ++sc) {
+        const bool wantS = any && sc >= cFirst / kDecSuper && sc <= cLast / kDecSuper &&
+                           !(boxDist(boxes + (maxChunks + sc) * 2 * DIM) > skipAbove);
+        if (__ballot(wantS) == 0) continue;
+        const int cEnd = min(wLast, sc * kDecSuper + kDecSuper - 1);
+        for (int c = max(wFirst, sc * kDecSuper); c <= cEnd; ++c) {
+            const bool wantC = wantS && c >= cFirst && c <= cLast && !(boxDist(boxes + (int64_t)c * 2 * DIM) > skipAbove);
+            if (__ballot(wantC) == 0) continue;
+            // E holds maxChunks * kDecChunk rows: a candidate past the end is read but never used
+#pragma unroll 4
+            for (int k = 0; k < kDecChunk; ++k) {
+                const int j = c * kDecChunk + k;
+                T q[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) q[d] = E[(int64_t)j * DIM + d];
+                if (wantC && j >= o && j < o + n) {
+                    const T dd = exact_sqdist<T, DIM>(p, q);
+                    const bool in1 = dd <= rSqr;
+                    const bool in2 = dd <= r4;
+                    c1 += in1;
+                    c2 += in2;
+                    if (fill && in2 && j != e) a.nbr[w++] = (j << 1) | (in1 ? 1 : 0);
+                }
+            }
         }
     }
-    if (!fill) {
+    if (!fill && valid) {
         a.c1[e] = c1;
         a.c2[e] = c2;
         // neighbours other than itself (scanned to offsets); elements of a skipped group
@@ -528,7 +595,9 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     hipLaunchKernelGGL(dec_elems_kernel, dim3(a.numSegs), dim3(256), 0, st, a);
     // 2. chunk boxes, counts
     const int64_t maxChunks = std::max<int64_t>((maxElems + kDecChunk - 1) / kDecChunk, 1);
-    const int64_t boxBytes = maxChunks * 2 * dim * (int64_t)sizeof(T);
+    const int64_t maxSupers = (maxChunks + kDecSuper - 1) / kDecSuper;
+    // chunk boxes, super boxes, then the element-order coordinates (maxChunks * kDecChunk rows)
+    const int64_t boxBytes = ((maxChunks + maxSupers) * 2 * dim + maxChunks * kDecChunk * dim) * (int64_t)sizeof(T);
     if (boxBytes > ws.boxCap) {
         if (ws.boxes) (void)hipFree(ws.boxes);
         ws.boxes = nullptr;
@@ -536,14 +605,17 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
         if ((err = hipMalloc(&ws.boxes, ws.boxCap)) != hipSuccess) return err;
     }
     T* boxes = static_cast<T*>(ws.boxes);
+    T* coords = boxes + (maxChunks + maxSupers) * 2 * dim;
     const unsigned blocks = (unsigned)((maxElems + 1 + 255) / 256);
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
         hipLaunchKernelGGL((dec_chunk_box_kernel<T, DIM>),
                            dim3((unsigned)((maxChunks + kBoxChunksPerBlock - 1) / kBoxChunksPerBlock)), dim3(256), 0, st, a,
-                           (const T*)S, sStride, boxes, maxChunks);
-        hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, S, sStride, groupRSqr,
-                           0, maxElems, (const T*)boxes);
+                           (const T*)S, sStride, boxes, maxChunks, coords);
+        hipLaunchKernelGGL((dec_super_box_kernel<T, DIM>), dim3((unsigned)((maxSupers + 255) / 256)), dim3(256), 0,
+                           st, boxes, maxChunks, maxSupers);
+        hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, (const T*)coords,
+                           groupRSqr, 0, maxElems, (const T*)boxes, maxChunks);
         ok = true;
     });
     if (!ok) return hipErrorInvalidValue;
@@ -561,8 +633,8 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     }
     a.nbr = ws.nbr;
     RD_DISPATCH_DIM(dim, {
-        hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, S, sStride, groupRSqr,
-                           1, maxElems, (const T*)boxes);
+        hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, (const T*)coords,
+                           groupRSqr, 1, maxElems, (const T*)boxes, maxChunks);
     });
     // 4. ordered removal: small groups one wave each; large groups in dependency rounds
     if ((err = dec_ordered_removal(a, maxElems, ws, st)) != hipSuccess) return err;
