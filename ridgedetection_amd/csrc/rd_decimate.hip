@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void dec_elems_kernel(DecArgs a) {
 // Bounding box of every chunk of kDecChunk consecutive elements: lo at [c][0, DIM), hi at
 // [c][DIM, 2 DIM).  kBoxLanes lanes per chunk, each folding every kBoxLanes-th element.  The
 // kernel also stages the coordinates in element order (E[e][0, DIM)) for the count.
-constexpr int kDecChunk = 16;
+constexpr int kDecChunk = 8;
 constexpr int kBoxLanes = kDecChunk < kWave ? kDecChunk : kWave;
 constexpr int kBoxChunksPerBlock = 256 / kBoxLanes;
 static_assert((kDecChunk & (kDecChunk - 1)) == 0 && kDecChunk >= 8, "kDecChunk: a power of two");
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void dec_chunk_box_kernel(DecArgs a, const T* 
 }
 
 // Boxes of kDecSuper consecutive chunks (the hull of their boxes), after the chunk boxes.
-constexpr int kDecSuper = 16;
+constexpr int kDecSuper = 32;
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void dec_super_box_kernel(T* __restrict__ boxes, int64_t maxChunks,
