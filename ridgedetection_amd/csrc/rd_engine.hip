@@ -53,13 +53,26 @@ __global__ void row_to_soa_kernel(const T* __restrict__ src, int64_t n, int dim,
         for (int d = 0; d < dim; ++d) dst[d * pStride + i] = src[i * stride + d];
 }
 
-// out[d][i] = P[d][pIndex[i]] for the n positions
-template <typename T>
-__global__ void gather_positions_kernel(const T* __restrict__ P, int64_t pStride, const int* __restrict__ pIndex,
-                                        int64_t n, int dim, T* __restrict__ out, int64_t outStride) {
+// rows[p][0, DIM) = P[0, DIM)[p]: the points row-major, so that a gather finds a point's
+// coordinates in one cache line
+template <typename T, int DIM>
+__global__ void points_to_rows_kernel(const T* __restrict__ P, int64_t pStride, int64_t n, T* __restrict__ rows) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) rows[i * DIM + d] = P[d * pStride + i];
+}
+
+// out[d][i] = rows[pIndex[i]][d] for the n positions
+template <typename T, int DIM>
+__global__ void gather_positions_kernel(const T* __restrict__ rows, const int* __restrict__ pIndex, int64_t n,
+                                        T* __restrict__ out, int64_t outStride) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int64_t src = pIndex[i];
-        for (int d = 0; d < dim; ++d) out[d * outStride + i] = P[d * pStride + src];
+        T v[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) v[d] = rows[src * DIM + d];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) out[d * outStride + i] = v[d];
     }
 }
 
@@ -363,9 +376,20 @@ void Engine<T>::preparePositions() {
     if (n <= 0) return;
     psegStride_ = round_up(n, 64);
     Pseg_.reserve((size_t)psegStride_ * dim_);
-    hipLaunchKernelGGL((gather_positions_kernel<T>), dim3(grid_for(n)), dim3(256), 0, st_, P_.get(), pStride_,
-                       pIndexBuf_.get(), n, dim_, Pseg_.get(), psegStride_);
+    const int64_t np = std::max<int64_t>(nPoints_, 1);
+    DevBuf<T> rows;
+    rows.reserve((size_t)np * dim_);
+    bool ok = false;
+    RD_DISPATCH_DIM(dim_, {
+        hipLaunchKernelGGL((points_to_rows_kernel<T, DIM>), dim3(grid_for(nPoints_)), dim3(256), 0, st_, P_.get(),
+                           pStride_, nPoints_, rows.get());
+        hipLaunchKernelGGL((gather_positions_kernel<T, DIM>), dim3(grid_for(n)), dim3(256), 0, st_, rows.get(),
+                           pIndexBuf_.get(), n, Pseg_.get(), psegStride_);
+        ok = true;
+    });
+    if (!ok) throw std::invalid_argument("unsupported dimension");
     check(hipGetLastError(), "gather_positions");
+    check(hipStreamSynchronize(st_), "gather_positions");   // rows is released on return
     psegValid_ = true;
 }
 
