@@ -727,7 +727,7 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
         check(hipMemcpyAsync(dmn.get(), hmn.data(), sizeof(T) * hmn.size(), hipMemcpyHostToDevice, st_), "boxes");
         check(hipMemcpyAsync(dmx.get(), hmx.data(), sizeof(T) * hmx.size(), hipMemcpyHostToDevice, st_), "boxes");
         const int words = (L + 63) / 64;
-        const int rowsPer = (int)std::max<int64_t>(1, std::min<int64_t>(L, (int64_t(8) << 20) / words));
+        const int rowsPer = (int)std::max<int64_t>(1, std::min<int64_t>(L, (int64_t(1) << 20) / words));   // 8 MB a block
         DevBuf<unsigned long long> dbits;
         dbits.reserve((size_t)rowsPer * words);
         std::vector<unsigned long long> hbits((size_t)rowsPer * words);

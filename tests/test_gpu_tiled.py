@@ -76,6 +76,19 @@ def test_trees_match_oracle(oracle, kind, case):
         _same_tree(c.tree(kind, r1, cap, tpd), oracle.tree(kind, P, r1, cap, tpd))
 
 
+def test_grouped_tree_many_leaves(oracle):
+    """10,000 grouped leaves: the device's all-pairs neighbour bit matrix is filled and read
+    back in more than one row block."""
+    rng = np.random.default_rng(77)
+    P = rng.random((200000, 2)).astype(np.float32)
+    r1, cap, tpd = 0.01, 100000, [100, 100]
+    print("case grouped tree 10k leaves", flush=True)
+    with _ctx(P) as c:
+        got = c.tree(1, r1, cap, tpd)
+    assert len(got) == 10000
+    _same_tree(got, oracle.tree(1, P, r1, cap, tpd))
+
+
 @pytest.mark.parametrize("alg", [0, 2, 5, 6])
 def test_tiled_drivers_match_oracle(oracle, alg):
     P = synth.helix3d(50000, seed=41, dtype=np.float64)
