@@ -149,8 +149,13 @@ __global__ __launch_bounds__(256) void dec_super_box_kernel(T* __restrict__ boxe
     }
 }
 
+// Row entries an element may keep from the count pass (provisional rows).
+constexpr int kDecProv = 32;
+
 // Neighbour counts within r2 and 2 r2 of every element in its group (fill = 0), then the
-// CSR rows (fill = 1).  A wave walks the union of its lanes' chunk ranges with wave-uniform
+// CSR rows (fill = 1).  With prov, the count pass also writes each element's first kDecProv
+// row entries to prov[e][.]; rows that fit are copied into place (dec_prov_copy_kernel) and
+// the fill pass walks only the elements with longer rows.  A wave walks the union of its lanes' chunk ranges with wave-uniform
 // indices, so candidate coordinates and boxes are broadcast loads; each lane tests only the
 // candidates of its own group, in increasing element order, and only in chunks whose box
 // passes its own test, as a walk of its own would.
@@ -158,12 +163,13 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __restrict__ E,
                                                         const T* __restrict__ groupRSqr, int fill,
                                                         int64_t maxElems, const T* __restrict__ boxes,
-                                                        int64_t maxChunks) {
+                                                        int64_t maxChunks, int* __restrict__ prov) {
     const int64_t total = *a.totalElems;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool valid = e < total;
-    if (!valid && !fill && e <= maxElems) a.rowOff[e] = 0;   // tail of the offsets scan
-    if (__ballot(valid) == 0) return;                        // the whole wave lies past the end
+    if (e >= total && !fill && e <= maxElems) a.rowOff[e] = 0;   // tail of the offsets scan
+    // the fill pass skips the rows the provisional copy already wrote
+    const bool valid = e < total && !(fill && prov && a.c2[e] - 1 <= kDecProv);
+    if (__ballot(valid) == 0) return;   // no element of the wave needs a walk
     int o = 0, n = 0;
     T rSqr = T(0), p[DIM];
 #pragma unroll
@@ -179,6 +185,7 @@ __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __re
     const T r4 = T(4) * rSqr;
     int c1 = 0, c2 = 0;
     int w = fill && valid ? a.rowOff[e] : 0;
+    int np = 0;   // row entries found (count pass)
     // Chunks whose box lies farther than 2 r2 hold no neighbour: the squared distance to a
     // box bounds every member's from below; the margin covers the rounding of both sums.  A
     // super box holds its chunks' boxes, and every step of the box distance is monotone, so
@@ -224,7 +231,14 @@ __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __re
                     const bool in2 = dd <= r4;
                     c1 += in1;
                     c2 += in2;
-                    if (fill && in2 && j != e) a.nbr[w++] = (j << 1) | (in1 ? 1 : 0);
+                    if (in2 && j != e) {
+                        const int v = (j << 1) | (in1 ? 1 : 0);
+                        if (fill)
+                            a.nbr[w++] = v;
+                        else if (prov && np < kDecProv)
+                            prov[e * kDecProv + np] = v;
+                        ++np;
+                    }
                 }
             }
         }
@@ -236,6 +250,19 @@ __global__ __launch_bounds__(256) void dec_count_kernel(DecArgs a, const T* __re
         // (n == 0) have no list at all
         a.rowOff[e] = n > 0 ? c2 - 1 : 0;
     }
+}
+
+// Copies the provisional rows that hold the whole row (c2 - 1 <= kDecProv) into the CSR.
+__global__ __launch_bounds__(256) void dec_prov_copy_kernel(DecArgs a, const int* __restrict__ prov) {
+    const int64_t total = *a.totalElems;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int g = a.elemGroup[e];
+    if (a.groupCnt[g] <= 0) return;   // skipped group: no row
+    const int cnt = a.c2[e] - 1;
+    if (cnt > kDecProv) return;
+    const int o = a.rowOff[e];
+    for (int k = 0; k < cnt; ++k) a.nbr[o + k] = prov[e * kDecProv + k];
 }
 
 __global__ __launch_bounds__(64) void dec_scan_kernel(DecArgs a) {
@@ -596,8 +623,13 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     // 2. chunk boxes, counts
     const int64_t maxChunks = std::max<int64_t>((maxElems + kDecChunk - 1) / kDecChunk, 1);
     const int64_t maxSupers = (maxChunks + kDecSuper - 1) / kDecSuper;
-    // chunk boxes, super boxes, then the element-order coordinates (maxChunks * kDecChunk rows)
-    const int64_t boxBytes = ((maxChunks + maxSupers) * 2 * dim + maxChunks * kDecChunk * dim) * (int64_t)sizeof(T);
+    // chunk boxes, super boxes, the element-order coordinates (maxChunks * kDecChunk rows),
+    // then the provisional rows when they fit in 256 MB
+    const int64_t coordBytes = ((maxChunks + maxSupers) * 2 * dim + maxChunks * kDecChunk * dim) * (int64_t)sizeof(T);
+    const int64_t provOff = (coordBytes + 255) / 256 * 256;
+    const int64_t provBytes = maxElems * kDecProv * (int64_t)sizeof(int);
+    const bool useProv = provBytes <= (int64_t(256) << 20);
+    const int64_t boxBytes = useProv ? provOff + provBytes : coordBytes;
     if (boxBytes > ws.boxCap) {
         if (ws.boxes) (void)hipFree(ws.boxes);
         ws.boxes = nullptr;
@@ -606,6 +638,7 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     }
     T* boxes = static_cast<T*>(ws.boxes);
     T* coords = boxes + (maxChunks + maxSupers) * 2 * dim;
+    int* prov = useProv ? reinterpret_cast<int*>(static_cast<char*>(ws.boxes) + provOff) : nullptr;
     const unsigned blocks = (unsigned)((maxElems + 1 + 255) / 256);
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
@@ -615,7 +648,7 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
         hipLaunchKernelGGL((dec_super_box_kernel<T, DIM>), dim3((unsigned)((maxSupers + 255) / 256)), dim3(256), 0,
                            st, boxes, maxChunks, maxSupers);
         hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, (const T*)coords,
-                           groupRSqr, 0, maxElems, (const T*)boxes, maxChunks);
+                           groupRSqr, 0, maxElems, (const T*)boxes, maxChunks, prov);
         ok = true;
     });
     if (!ok) return hipErrorInvalidValue;
@@ -632,9 +665,12 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
         if ((err = hipMalloc(&ws.nbr, sizeof(int) * ws.nbrCap)) != hipSuccess) return err;
     }
     a.nbr = ws.nbr;
+    if (prov)
+        hipLaunchKernelGGL(dec_prov_copy_kernel, dim3((unsigned)((maxElems + 255) / 256)), dim3(256), 0, st, a,
+                           (const int*)prov);
     RD_DISPATCH_DIM(dim, {
         hipLaunchKernelGGL((dec_count_kernel<T, DIM>), dim3(blocks), dim3(256), 0, st, a, (const T*)coords,
-                           groupRSqr, 1, maxElems, (const T*)boxes, maxChunks);
+                           groupRSqr, 1, maxElems, (const T*)boxes, maxChunks, prov);
     });
     // 4. ordered removal: small groups one wave each; large groups in dependency rounds
     if ((err = dec_ordered_removal(a, maxElems, ws, st)) != hipSuccess) return err;
