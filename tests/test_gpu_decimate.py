@@ -44,6 +44,24 @@ def test_large_group_matches_oracle(oracle, K, dim, r2):
     np.testing.assert_array_equal(np.flatnonzero(keep), kept)
 
 
+@pytest.mark.parametrize("K,dim,r2", [(6000, 3, 0.04), (5000, 12, 0.2)])
+def test_large_group_fp64_matches_oracle(oracle, K, dim, r2):
+    """FP64 groups: the count pass, the provisional and walked rows and the rounds in double."""
+    S = _noisy_curve(K, min(dim, 4), seed=3 * K + dim, sigma=0.03).astype(np.float64)
+    if dim > S.shape[1]:   # the remaining coordinates: small noise around 0
+        g = np.random.Generator(np.random.PCG64(dim))
+        S = np.concatenate([S, g.normal(0.0, 0.01, (K, dim - S.shape[1]))], axis=1)
+    S = np.ascontiguousarray(S)
+    _say(f"case decimate fp64 K={K} d={dim} r2={r2}")
+    with _ctx(dim, np.float64) as c:
+        c.set_points(S)
+        c.set_chosen(S)
+        SD, keep = c.decimate(r2, rule=0)
+    ref, kept = oracle.decimate(S, r2)
+    np.testing.assert_array_equal(SD, ref)
+    np.testing.assert_array_equal(np.flatnonzero(keep), kept)
+
+
 @pytest.mark.parametrize("K,r2", [(3000, 0.05), (8000, 0.12)])
 def test_large_group_nocopy_matches_oracle(oracle, K, r2):
     S = _noisy_curve(K, 3, seed=7 * K, sigma=0.03)
