@@ -76,6 +76,16 @@ def test_trees_match_oracle(oracle, kind, case):
         _same_tree(c.tree(kind, r1, cap, tpd), oracle.tree(kind, P, r1, cap, tpd))
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_trees_fp64_match_oracle(oracle, kind):
+    """FP64 trees: split bounds, cell tables and the grouped neighbour matrix in double."""
+    P = synth.bezier_union(30000, dim=4, seed=33, dtype=np.float64)
+    r1, tpd, cap = 0.15, [3, 3, 3, 3], 900
+    print(f"case tree fp64 kind {kind}", flush=True)
+    with _ctx(P) as c:
+        _same_tree(c.tree(kind, r1, cap, tpd), oracle.tree(kind, P, r1, cap, tpd))
+
+
 def test_grouped_tree_many_leaves(oracle):
     """10,000 grouped leaves: the device's all-pairs neighbour bit matrix is filled and read
     back in more than one row block."""
