@@ -44,6 +44,19 @@ def test_large_group_matches_oracle(oracle, K, dim, r2):
     np.testing.assert_array_equal(np.flatnonzero(keep), kept)
 
 
+@pytest.mark.parametrize("K", [1, 2, 3, 5, 9, 17, 33, 65, 257])
+def test_small_groups_match_oracle(oracle, K):
+    """Groups that end inside a pruning chunk or a wave: the count pass reads candidates past
+    the last element (never used) and lanes past the end stay out of every row."""
+    g = np.random.Generator(np.random.PCG64(K))
+    S = np.ascontiguousarray(g.normal(0.0, 0.05, (K, 3)).astype(np.float32))
+    _say(f"case decimate small K={K}")
+    SD, keep = _decimate(S, 0.04, rule=0)
+    ref, kept = oracle.decimate(S, 0.04)
+    np.testing.assert_array_equal(SD, ref)
+    np.testing.assert_array_equal(np.flatnonzero(keep), kept)
+
+
 @pytest.mark.parametrize("K,dim,r2", [(6000, 3, 0.04), (5000, 12, 0.2)])
 def test_large_group_fp64_matches_oracle(oracle, K, dim, r2):
     """FP64 groups: the count pass, the provisional and walked rows and the rounds in double."""
