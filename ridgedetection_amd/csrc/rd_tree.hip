@@ -758,22 +758,32 @@ void TileTreeBuilder<T>::build(const T* P, int64_t ps, int64_t n, int kind, T r1
 
 template <typename T>
 void TileTreeBuilder<T>::writeIndex(int* dIndex) {
-    std::vector<RangeCopy> rc;
-    int64_t dst = 0;
     const int L = (int)leaves_.size();
+    // every leaf's own range, compact (a grouped layout repeats each one for every neighbour)
+    std::vector<RangeCopy> own(L);
+    size_t ranges = (size_t)L;
     for (int l = 0; l < L; ++l) {
         const HTile& t = tiles_[leafTiles_[l]];
-        rc.push_back({bufs_[t.ownBuf]->get(), t.ownOff, t.ownCnt, dst});
-        dst += t.ownCnt;
-        if (kind_ == TREE_EXTENDED && t.nbCnt > 0) {
-            rc.push_back({bufs_[t.nbBuf]->get(), t.nbOff, t.nbCnt, dst});
-            dst += t.nbCnt;
+        own[l] = {bufs_[t.ownBuf]->get(), t.ownOff, t.ownCnt, 0};
+        ranges += kind_ == TREE_GROUPED ? leaves_[l].nbLeaves.size() : 1;
+    }
+    std::vector<RangeCopy> rc;
+    rc.reserve(ranges);
+    int64_t dst = 0;
+    for (int l = 0; l < L; ++l) {
+        rc.push_back({own[l].src, own[l].srcOff, own[l].cnt, dst});
+        dst += own[l].cnt;
+        if (kind_ == TREE_EXTENDED) {
+            const HTile& t = tiles_[leafTiles_[l]];
+            if (t.nbCnt > 0) {
+                rc.push_back({bufs_[t.nbBuf]->get(), t.nbOff, t.nbCnt, dst});
+                dst += t.nbCnt;
+            }
         }
         if (kind_ == TREE_GROUPED)
             for (int j : leaves_[l].nbLeaves) {
-                const HTile& o = tiles_[leafTiles_[j]];
-                rc.push_back({bufs_[o.ownBuf]->get(), o.ownOff, o.ownCnt, dst});
-                dst += o.ownCnt;
+                rc.push_back({own[j].src, own[j].srcOff, own[j].cnt, dst});
+                dst += own[j].cnt;
             }
     }
     if (rc.empty()) return;
