@@ -28,7 +28,11 @@
  *     ROW_MAJOR: point i at p[i*stride + d] (stride >= dim, 0 = dim);
  *     COL_MAJOR / SOA: point i at p[d*stride + i] (stride >= n, 0 = n).
  *   - Radii: r1 is the choose/evolve radius, r2 the decimate radius (reference names).
- *   - max_evolve_iters = 0 runs every evolve to convergence, as the reference does.
+ *   - max_evolve_iters = 0 runs every evolve to convergence, as the reference does, up to an
+ *     iteration guard (RDHIP_OPT_EVOLVE_GUARD, 100000 by default, 0 = none) past which the call
+ *     fails with RDHIP_ERR_RUNTIME rather than looping on.
+ *   - After RDHIP_ERR_HIP (a failed allocation, launch or copy) a context is unusable: every
+ *     later call on it returns RDHIP_ERR_STATE until rdhip_destroy.
  */
 #ifndef RDHIP_H
 #define RDHIP_H
@@ -122,9 +126,16 @@ enum rdhip_option {
                                       plus one workgroup per segment for the member sort, ordered
                                       sums and shift; 0 (default, faster on the measured tiled and
                                       batched workloads): batch-wide sort and sums */
-    RDHIP_OPT_POSITION_COPY = 11   /* 1 (default): tiled runs copy the samples into segment-position
+    RDHIP_OPT_POSITION_COPY = 11,  /* 1 (default): tiled runs copy the samples into segment-position
                                       order once per tiling, so the evolve and choose passes read them
                                       without the position -> point index; 0: read through the index */
+    RDHIP_OPT_EVOLVE_GUARD = 12,   /* an evolve that still moves after this many iterations stops
+                                      with RDHIP_ERR_RUNTIME instead of running on (default 100000;
+                                      0 = no limit, the reference's behaviour, which never returns
+                                      on an input that does not converge) */
+    RDHIP_OPT_INJECT_ALLOC_FAILURE = 13  /* test hook: the value-th device allocation the calling
+                                      thread makes from now on (in any context) fails with
+                                      hipErrorOutOfMemory; 0 = off */
 };
 
 int rdhip_version(void);
@@ -143,6 +154,14 @@ int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, 
  * chosen points is stable (rd/cpu/brute_force/ridge_detection.inl:67-146). */
 int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evolve_iters,
                           int64_t* out_count);
+
+/* Brute-force ridge detection over a list of sample sets, concatenated in the context's
+ * points: choose takes the first n_own points (the first set) only, evolve takes all of them,
+ * own points first -- rd::RidgeDetection<T>::ridgeDetection(std::vector<std::pair<T const*,
+ * size_t>> samples, ...) (rd/cpu/brute_force/ridge_detection.inl:171-286).  n_own < 0: all
+ * points (= rdhip_ridge_detection). */
+int rdhip_ridge_detection_own(rdhip_ctx* ctx, int64_t n_own, double r1, double r2, int64_t max_evolve_iters,
+                              int64_t* out_count);
 
 /* Batched brute-force ridge detection: `num_clouds` independent clouds of the context's
  * dimension (host ROW_MAJOR pointers, sizes[c] points, radii r1[c] / r2[c]) each run the
@@ -242,8 +261,16 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out);
  * chosen points' input indices in ascending order. */
 int rdhip_choose(rdhip_ctx* ctx, double r, int64_t* out_count, int32_t* out_indices);
 
-/* Replaces the chosen set (host pointer, `count` points in `layout`). */
+/* Replaces the chosen set (host pointer, `count` points in `layout`, packed).  At most as many
+ * points as the cloud holds. */
 int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout);
+
+/* The same from host or device memory (on_device) with a stride (0 = packed): ROW_MAJOR stride
+ * >= dim between points, COL_MAJOR / SOA stride >= count between coordinates.  The input form
+ * of DeviceEvolve::evolve's d_chosenSamples and DeviceDecimate::decimate's d_chosenPoints
+ * (rd/gpu/device/device_evolve.cuh:35-62, device_decimate.cuh:55-70). */
+int rdhip_set_chosen_ex(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout, int64_t stride,
+                        int on_device);
 
 /* Evolve the chosen set with radius r until convergence (or max_iters > 0 iterations). */
 int rdhip_evolve(rdhip_ctx* ctx, double r, int64_t max_iters, int64_t* out_iters);

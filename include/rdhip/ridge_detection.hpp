@@ -10,6 +10,10 @@
 //   * TileType gains RD_LOCAL_TILE and RD_GROUPED_TILE (the reference GPU only has
 //     RD_EXTENDED_TILE; the CPU reference has all three trees);
 //   * RD_KD_TREE / RD_BVH_TREE are declared but, as in the reference, not implemented;
+//   * the device facades DeviceChoose / DeviceEvolve / DeviceDecimate (namespace
+//     rd::gpu::bruteForce, ridge_detection/rd/gpu/device/device_{choose,evolve,decimate}.cuh)
+//     return hipError_t; the library's message for a failure is rdhip_last_error().  They run
+//     synchronously on the given stream, each call on a context of its own;
 //   * rd::gpu::SamplesGenerator<T> (ridge_detection/rd/gpu/device/samples_generator.cuh:
 //     291-384) draws from librdhip's counter-based generator (seed 1234, as the reference's
 //     curand_init), so its samples follow the reference's distributions but are not the
@@ -28,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 namespace rd {
 
@@ -117,26 +122,40 @@ public:
     size_t pPitch_;    // COL_MAJOR: row pitch of dP_ in bytes; otherwise 0
     T* dS_;            // chosen points in device memory, OUT_MEM_LAYOUT
     size_t sPitch_;    // COL_MAJOR: row pitch of dS_ in bytes; otherwise 0
+    int* dNs_;         // device copy of ns_ (the reference's rdBruteForceNs symbol)
+    // The reference's evolve / decimate work buffers.  The library keeps its own, so these
+    // stay null; they are declared for source compatibility.
+    T* dCordSums_ = nullptr;
+    size_t csPitch_ = 0;
+    int* dSpherePointCnt_ = nullptr;
+    T* dDistMtx_ = nullptr;
+    size_t distMtxPitch_ = 0;
+    char* dPtsMask_ = nullptr;
     bool verbose_;
     long long maxEvolveIters_ = 0;   // 0: evolve to convergence (reference behaviour)
 
     RidgeDetection(size_t np, T r1, T r2, bool verbose = false)
-        : np_(np), ns_(0), r1_(r1), r2_(r2), dP_(nullptr), pPitch_(0), dS_(nullptr), sPitch_(0), verbose_(verbose) {
-        detail::check(rdhip_create(&ctx_, detail::dtype<T>(), DIM, nullptr), "rdhip_create");
-        if (IN_MEM_LAYOUT == COL_MAJOR)
-            detail::checkHip(hipMallocPitch((void**)&dP_, &pPitch_, np_ * sizeof(T), DIM), "hipMallocPitch");
-        else
-            detail::checkHip(hipMalloc((void**)&dP_, np_ * DIM * sizeof(T)), "hipMalloc");
-        if (OUT_MEM_LAYOUT == COL_MAJOR)
-            detail::checkHip(hipMallocPitch((void**)&dS_, &sPitch_, np_ * sizeof(T), DIM), "hipMallocPitch");
-        else
-            detail::checkHip(hipMalloc((void**)&dS_, np_ * DIM * sizeof(T)), "hipMalloc");
+        : np_(np), ns_(0), r1_(r1), r2_(r2), dP_(nullptr), pPitch_(0), dS_(nullptr), sPitch_(0), dNs_(nullptr),
+          verbose_(verbose) {
+        static_assert(IN_MEM_LAYOUT != SOA && OUT_MEM_LAYOUT != SOA, "ROW_MAJOR or COL_MAJOR (as in the reference)");
+        try {
+            detail::check(rdhip_create(&ctx_, detail::dtype<T>(), DIM, nullptr), "rdhip_create");
+            detail::checkHip(hipMalloc((void**)&dNs_, sizeof(int)), "hipMalloc");
+            detail::checkHip(hipMemset(dNs_, 0, sizeof(int)), "hipMemset");
+            if (IN_MEM_LAYOUT == COL_MAJOR)
+                detail::checkHip(hipMallocPitch((void**)&dP_, &pPitch_, np_ * sizeof(T), DIM), "hipMallocPitch");
+            else
+                detail::checkHip(hipMalloc((void**)&dP_, np_ * DIM * sizeof(T)), "hipMalloc");
+            if (OUT_MEM_LAYOUT == COL_MAJOR)
+                detail::checkHip(hipMallocPitch((void**)&dS_, &sPitch_, np_ * sizeof(T), DIM), "hipMallocPitch");
+            else
+                detail::checkHip(hipMalloc((void**)&dS_, np_ * DIM * sizeof(T)), "hipMalloc");
+        } catch (...) {
+            release();
+            throw;
+        }
     }
-    virtual ~RidgeDetection() {
-        if (dP_) (void)hipFree(dP_);
-        if (dS_) (void)hipFree(dS_);
-        rdhip_destroy(ctx_);
-    }
+    virtual ~RidgeDetection() { release(); }
     RidgeDetection(const RidgeDetection&) = delete;
     RidgeDetection& operator=(const RidgeDetection&) = delete;
 
@@ -154,16 +173,203 @@ public:
         if (ns_ > 0)
             detail::check(rdhip_get_points(ctx_, dS_, detail::layout(OUT_MEM_LAYOUT), outStride, 1),
                           "rdhip_get_points");
+        const int ns32 = (int)ns_;
+        detail::checkHip(hipMemcpy(dNs_, &ns32, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
     }
+    // ns_ from the device copy dNs_ (simulation.inl: getChosenSamplesCount)
     void getChosenSamplesCount() {
-        int64_t ns = 0;
-        detail::check(rdhip_get_count(ctx_, &ns), "rdhip_get_count");
+        int ns = 0;
+        detail::checkHip(hipMemcpy(&ns, dNs_, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
         ns_ = (size_t)ns;
     }
     rdhip_ctx* context() const { return ctx_; }
 
 private:
     rdhip_ctx* ctx_ = nullptr;
+
+    void release() {
+        if (dP_) (void)hipFree(dP_);
+        if (dS_) (void)hipFree(dS_);
+        if (dNs_) (void)hipFree(dNs_);
+        dP_ = dS_ = nullptr;
+        dNs_ = nullptr;
+        rdhip_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+};
+
+// ---- device-level facades ------------------------------------------------------------
+// The reference's DeviceChoose / DeviceEvolve / DeviceDecimate structs with their template
+// parameters and argument lists.  Pointers are device memory; strides count elements between
+// consecutive points (ROW_MAJOR, usually DIM) or between coordinate rows (COL_MAJOR, the pitch
+// in elements).  Results follow the reference's serial CPU path (choose order, first-index
+// ties, summation in sample order), so they do not vary between runs.
+namespace facade_detail {
+inline hipError_t toHip(int rc) {
+    return rc == RDHIP_OK ? hipSuccess : rc == RDHIP_ERR_INVALID ? hipErrorInvalidValue : hipErrorUnknown;
+}
+// A context on `stream` for one facade call (destroyed, after a stream synchronisation, on exit).
+struct ScopedCtx {
+    rdhip_ctx* c = nullptr;
+    int rc = RDHIP_OK;
+    ScopedCtx(int dtype, int dim, hipStream_t stream) { rc = rdhip_create(&c, dtype, dim, (void*)stream); }
+    ~ScopedCtx() { rdhip_destroy(c); }
+    ScopedCtx(const ScopedCtx&) = delete;
+    ScopedCtx& operator=(const ScopedCtx&) = delete;
+};
+}  // namespace facade_detail
+
+// rd/gpu/device/device_choose.cuh:55-82: greedy choose of r-separated points from
+// d_inputPoints into d_chosenPoints (OUT_MEM_LAYOUT); the count goes to *d_chosenPointsNum.
+struct DeviceChoose {
+    template <int DIM, DataMemoryLayout IN_MEM_LAYOUT, DataMemoryLayout OUT_MEM_LAYOUT, typename T>
+    static hipError_t choose(T const* d_inputPoints, T* d_chosenPoints, int inputPointsNum, int* d_chosenPointsNum,
+                             T r, int inPtsStride, int chosenPtsStride, hipStream_t stream = nullptr,
+                             bool debugSynchronous = false) {
+        (void)debugSynchronous;
+        facade_detail::ScopedCtx x(gpu::detail::dtype<T>(), DIM, stream);
+        int rc = x.rc;
+        int64_t K = 0;
+        if (rc == RDHIP_OK)
+            rc = rdhip_set_points(x.c, d_inputPoints, inputPointsNum, gpu::detail::layout(IN_MEM_LAYOUT), inPtsStride, 1);
+        if (rc == RDHIP_OK) rc = rdhip_choose(x.c, (double)r, &K, nullptr);
+        if (rc == RDHIP_OK && K > 0)
+            rc = rdhip_get_points(x.c, d_chosenPoints, gpu::detail::layout(OUT_MEM_LAYOUT), chosenPtsStride, 1);
+        if (rc != RDHIP_OK) return facade_detail::toHip(rc);
+        const int k32 = (int)K;
+        hipError_t e = hipMemcpyAsync(d_chosenPointsNum, &k32, sizeof(int), hipMemcpyHostToDevice, stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(stream);
+    }
+};
+
+// rd/gpu/device/device_evolve.cuh:29-97: evolve d_chosenSamples (in place, OUTPUT_MEM_LAYOUT)
+// until no coordinate moves.  d_cordSums (OUTPUT_MEM_LAYOUT, stride cordSumsStride) and
+// d_spherePointCount receive the last iteration's coordinate sums and member counts, as the
+// reference's work buffers hold them on return.  The second overload evolves over the samples
+// followed by the neighbour samples (a tile's own and neighbouring points).
+struct DeviceEvolve {
+    template <int DIM, DataMemoryLayout INPUT_MEM_LAYOUT, DataMemoryLayout OUTPUT_MEM_LAYOUT, typename T>
+    static hipError_t evolve(T const* d_inputPoints, T* d_chosenSamples, T* d_cordSums, int* d_spherePointCount,
+                             int inputPointsNum, int chosenPointsNum, T r, int inPtsStride, int chosenPtsStride,
+                             int cordSumsStride, hipStream_t stream = nullptr, bool debugSynchronous = false) {
+        (void)debugSynchronous;
+        facade_detail::ScopedCtx x(gpu::detail::dtype<T>(), DIM, stream);
+        int rc = x.rc;
+        if (rc == RDHIP_OK)
+            rc = rdhip_set_points(x.c, d_inputPoints, inputPointsNum, gpu::detail::layout(INPUT_MEM_LAYOUT),
+                                  inPtsStride, 1);
+        return finish<DIM, OUTPUT_MEM_LAYOUT>(x, rc, d_chosenSamples, d_cordSums, d_spherePointCount, chosenPointsNum,
+                                              r, chosenPtsStride, cordSumsStride, stream);
+    }
+
+    template <int DIM, DataMemoryLayout INPUT_MEM_LAYOUT, DataMemoryLayout OUTPUT_MEM_LAYOUT, typename T>
+    static hipError_t evolve(T const* d_inputPoints, T const* d_neighbourPoints, T* d_chosenSamples, T* d_cordSums,
+                             int* d_spherePointCount, int inputPointsNum, int neighbourPointsNum, int chosenPointsNum,
+                             T r, int inPtsStride, int neighbourPtsStride, int chosenPtsStride, int cordSumsStride,
+                             hipStream_t stream = nullptr, bool debugSynchronous = false) {
+        (void)debugSynchronous;
+        // own samples, then the neighbour samples, in one buffer of the input layout
+        const size_t n = (size_t)inputPointsNum + (size_t)neighbourPointsNum;
+        T* all = nullptr;
+        hipError_t e = hipMalloc((void**)&all, sizeof(T) * (n > 0 ? n : 1) * DIM);
+        if (e != hipSuccess) return e;
+        const bool row = INPUT_MEM_LAYOUT == ROW_MAJOR;
+        const size_t rowB = sizeof(T) * DIM;
+        if (inputPointsNum > 0)
+            e = row ? hipMemcpy2DAsync(all, rowB, d_inputPoints, sizeof(T) * inPtsStride, rowB, inputPointsNum,
+                                       hipMemcpyDeviceToDevice, stream)
+                    : hipMemcpy2DAsync(all, sizeof(T) * n, d_inputPoints, sizeof(T) * inPtsStride,
+                                       sizeof(T) * inputPointsNum, DIM, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess && neighbourPointsNum > 0)
+            e = row ? hipMemcpy2DAsync(all + (size_t)inputPointsNum * DIM, rowB, d_neighbourPoints,
+                                       sizeof(T) * neighbourPtsStride, rowB, neighbourPointsNum, hipMemcpyDeviceToDevice,
+                                       stream)
+                    : hipMemcpy2DAsync(all + inputPointsNum, sizeof(T) * n, d_neighbourPoints,
+                                       sizeof(T) * neighbourPtsStride, sizeof(T) * neighbourPointsNum, DIM,
+                                       hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(all);
+            return e;
+        }
+        facade_detail::ScopedCtx x(gpu::detail::dtype<T>(), DIM, stream);
+        int rc = x.rc;
+        if (rc == RDHIP_OK)
+            rc = rdhip_set_points(x.c, all, (int64_t)n, row ? RDHIP_ROW_MAJOR : RDHIP_SOA, 0, 1);
+        (void)hipFree(all);   // set_points copied the samples
+        return finish<DIM, OUTPUT_MEM_LAYOUT>(x, rc, d_chosenSamples, d_cordSums, d_spherePointCount, chosenPointsNum,
+                                              r, chosenPtsStride, cordSumsStride, stream);
+    }
+
+private:
+    template <int DIM, DataMemoryLayout OUT, typename T>
+    static hipError_t finish(facade_detail::ScopedCtx& x, int rc, T* dS, T* dSums, int* dCounts, int ns, T r,
+                             int sStride, int csStride, hipStream_t stream) {
+        const int lay = gpu::detail::layout(OUT);
+        if (rc == RDHIP_OK) rc = rdhip_set_chosen_ex(x.c, dS, ns, lay, sStride, 1);
+        if (rc == RDHIP_OK) rc = rdhip_evolve(x.c, (double)r, 0, nullptr);
+        if (rc == RDHIP_OK && ns > 0) rc = rdhip_get_points(x.c, dS, lay, sStride, 1);
+        if (rc != RDHIP_OK || ns == 0) return facade_detail::toHip(rc);
+        // the converged set is a fixed point: one more pass yields the last pass's sums and counts
+        std::vector<int> counts((size_t)ns);
+        std::vector<T> sums((size_t)ns * DIM);
+        int flag = 0;
+        rc = rdhip_evolve_iteration(x.c, (double)r, nullptr, counts.data(), sums.data(), nullptr, &flag);
+        if (rc != RDHIP_OK) return facade_detail::toHip(rc);
+        hipError_t e = hipMemcpyAsync(dCounts, counts.data(), sizeof(int) * ns, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess && dSums) {
+            if (OUT == ROW_MAJOR)
+                e = hipMemcpy2DAsync(dSums, sizeof(T) * csStride, sums.data(), sizeof(T) * DIM, sizeof(T) * DIM, ns,
+                                     hipMemcpyHostToDevice, stream);
+            else {
+                std::vector<T> cm((size_t)ns * DIM);
+                for (int k = 0; k < ns; ++k)
+                    for (int d = 0; d < DIM; ++d) cm[(size_t)d * ns + k] = sums[(size_t)k * DIM + d];
+                e = hipMemcpy2DAsync(dSums, sizeof(T) * csStride, cm.data(), sizeof(T) * ns, sizeof(T) * ns, DIM,
+                                     hipMemcpyHostToDevice, stream);
+            }
+        }
+        return e != hipSuccess ? e : hipStreamSynchronize(stream);
+    }
+};
+
+// rd/gpu/device/device_decimate.cuh:50-117: decimation of d_chosenPoints (in place, stable)
+// with radius r; *chosenPointsNum (device memory) holds the count on entry and on return.
+// decimateDistMtx applies the same rule; its distance-matrix and mask work buffers are not
+// needed and are left untouched.
+struct DeviceDecimate {
+    template <int DIM, DataMemoryLayout MEM_LAYOUT, typename T>
+    static hipError_t decimate(T* d_chosenPoints, int* chosenPointsNum, T r, int stride, hipStream_t stream = nullptr,
+                               bool debug_synchronous = false) {
+        (void)debug_synchronous;
+        int K = 0;
+        hipError_t e = hipMemcpyAsync(&K, chosenPointsNum, sizeof(int), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return e;
+        if (K <= 0) return hipSuccess;
+        facade_detail::ScopedCtx x(gpu::detail::dtype<T>(), DIM, stream);
+        const int lay = gpu::detail::layout(MEM_LAYOUT);
+        int rc = x.rc;
+        int64_t left = 0;
+        if (rc == RDHIP_OK) rc = rdhip_set_points(x.c, d_chosenPoints, K, lay, stride, 1);
+        if (rc == RDHIP_OK) rc = rdhip_set_chosen_ex(x.c, d_chosenPoints, K, lay, stride, 1);
+        if (rc == RDHIP_OK) rc = rdhip_decimate(x.c, (double)r, RDHIP_DECIMATE, nullptr, &left);
+        if (rc == RDHIP_OK && left > 0) rc = rdhip_get_points(x.c, d_chosenPoints, lay, stride, 1);
+        if (rc != RDHIP_OK) return facade_detail::toHip(rc);
+        const int k32 = (int)left;
+        e = hipMemcpyAsync(chosenPointsNum, &k32, sizeof(int), hipMemcpyHostToDevice, stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(stream);
+    }
+
+    template <int DIM, DataMemoryLayout MEM_LAYOUT, typename T>
+    static hipError_t decimateDistMtx(T* d_chosenPoints, int* chosenPointsNum, int chPtsStride, T* d_distMtx,
+                                      int distMtxStride, char* d_pointsMask, T r, hipStream_t stream = nullptr,
+                                      bool debug_synchronous = false) {
+        (void)d_distMtx;
+        (void)distMtxStride;
+        (void)d_pointsMask;
+        return decimate<DIM, MEM_LAYOUT, T>(d_chosenPoints, chosenPointsNum, r, chPtsStride, stream, debug_synchronous);
+    }
 };
 
 }  // namespace bruteForce
@@ -189,9 +395,12 @@ class RidgeDetection {
     static_assert(RD_TILE_ALGORITHM == RD_BRUTE_FORCE, "only RD_BRUTE_FORCE is implemented (as in the reference)");
 
 public:
-    // inputPoints: host memory, ROW_MAJOR (as in the reference)
+    // inputPoints: host memory, ROW_MAJOR (as in the reference, whatever IN_MEM_LAYOUT, which
+    // only names the device copy's layout); SOA throws std::runtime_error as the reference does
+    // (rd/gpu/device/tiled/simulation.cuh:100-117)
     RidgeDetection(int pointsNum, T const* inputPoints, bool enableTiming = false, bool debugSynchronous = false)
         : pointsNum_(pointsNum), enableTiming_(enableTiming), debugSynchronous_(debugSynchronous) {
+        if (IN_MEM_LAYOUT != ROW_MAJOR && IN_MEM_LAYOUT != COL_MAJOR) throw std::runtime_error("Unsupported memory layout!");
         gpu::detail::check(rdhip_create(&ctx_, gpu::detail::dtype<T>(), DIM, nullptr), "rdhip_create");
         gpu::detail::check(rdhip_set_points(ctx_, inputPoints, pointsNum, RDHIP_ROW_MAJOR, 0, 0), "rdhip_set_points");
     }
@@ -211,10 +420,13 @@ public:
         chosenPointsNum_ = (int)n;
         return chosenPointsNum_;
     }
-    // chosenPoints: host memory for getChosenPointsNum() points, in OUT_MEM_LAYOUT
+    // chosenPoints: host memory for getChosenPointsNum() points, always written ROW_MAJOR
+    // (OUT_MEM_LAYOUT names the device layout, which the reference transposes on the way
+    // back); SOA throws std::runtime_error (rd/gpu/device/tiled/simulation.cuh:174-203)
     void getChosenPoints(T* chosenPoints) {
-        const int layout = OUT_MEM_LAYOUT == ROW_MAJOR ? RDHIP_ROW_MAJOR : RDHIP_SOA;
-        gpu::detail::check(rdhip_get_points(ctx_, chosenPoints, layout, 0, 0), "rdhip_get_points");
+        if (OUT_MEM_LAYOUT != ROW_MAJOR && OUT_MEM_LAYOUT != COL_MAJOR) throw std::runtime_error("Unsupported memory layout!");
+        if (chosenPointsNum_ == 0) return;
+        gpu::detail::check(rdhip_get_points(ctx_, chosenPoints, RDHIP_ROW_MAJOR, 0, 0), "rdhip_get_points");
     }
     gpu::detail::TiledRidgeDetectionTimers getTimers() {
         rdhip_timers t{};

@@ -258,6 +258,17 @@ class Oracle:
                                      self._scalar(P.dtype, r2), _p(S), C.c_int(1 if omp else 0))
         return S[:k].copy(), -1
 
+    def ridge_detection_own(self, P, n_own, r1, r2, max_evolve_iter=0):
+        """The sample-list overload: choose over P[:n_own], evolve over all of P (restatement only)."""
+        P = np.ascontiguousarray(P)
+        n, d = P.shape
+        S = np.empty_like(P)
+        rounds = C.c_long(0)
+        k = self._check(self._fn("rd_own", P.dtype)(_p(P), C.c_long(n), C.c_long(n_own), C.c_int(d),
+                                                     self._scalar(P.dtype, r1), self._scalar(P.dtype, r2),
+                                                     C.c_long(max_evolve_iter), _p(S), C.byref(rounds)), "rd_own")
+        return S[:k].copy(), int(rounds.value)
+
     def tiled(self, alg, P, r1, r2, max_tile_capacity, tiles_per_dim, refinement=False,
               max_evolve_iter=0, threads=1):
         P = np.ascontiguousarray(P)

@@ -855,8 +855,8 @@ long chooseC(const T* P, long n, int dim, T r, T* S, long* idx) {
         std::vector<T> s;
         std::vector<long> ci;
         long K = oracle::choose<T>(P, nullptr, n, dim, r, s, &ci);
-        if (S) std::memcpy(S, s.data(), sizeof(T) * s.size());
-        if (idx) std::memcpy(idx, ci.data(), sizeof(long) * ci.size());
+        if (S && !s.empty()) std::memcpy(S, s.data(), sizeof(T) * s.size());
+        if (idx && !ci.empty()) std::memcpy(idx, ci.data(), sizeof(long) * ci.size());
         return K;
     });
 }
@@ -903,7 +903,7 @@ long decimateC(T* S, long K, int dim, T r2, int noCopyRule, long* kept) {
                 if (kept) kept[j] = k;
                 ++j;
             }
-        std::memcpy(S, keep.data(), sizeof(T) * keep.size());
+        if (!keep.empty()) std::memcpy(S, keep.data(), sizeof(T) * keep.size());
         return nk;
     });
 }
@@ -915,7 +915,24 @@ long rdC(const T* P, long n, int dim, T r1, T r2, long maxEvolveIter, T* S, long
         std::vector<std::pair<const long*, long>> blocks{{nullptr, n}};
         std::vector<T> s;
         long K = oracle::ridgeDetection<T>(P, blocks, dim, r1, r2, maxEvolveIter, s, rounds);
-        std::memcpy(S, s.data(), sizeof(T) * s.size());
+        if (!s.empty()) std::memcpy(S, s.data(), sizeof(T) * s.size());
+        return K;
+    });
+}
+
+// The sample-list overload (rd/cpu/brute_force/ridge_detection.inl:171-286) with two sample
+// sets: the first nOwn points (choose and evolve) and the rest (evolve only).
+template <typename T>
+long rdOwnC(const T* P, long n, long nOwn, int dim, T r1, T r2, long maxEvolveIter, T* S, long* rounds) {
+    return guarded([&]() -> long {
+        if (!P || n < 1 || dim < 1 || nOwn < 1 || nOwn > n) throw std::runtime_error("ridge detection: invalid arguments");
+        std::vector<long> rest((size_t)(n - nOwn));
+        for (long i = nOwn; i < n; ++i) rest[(size_t)(i - nOwn)] = i;
+        std::vector<std::pair<const long*, long>> blocks{{nullptr, nOwn}};
+        if (n > nOwn) blocks.push_back({rest.data(), n - nOwn});
+        std::vector<T> s;
+        long K = oracle::ridgeDetection<T>(P, blocks, dim, r1, r2, maxEvolveIter, s, rounds);
+        if (!s.empty()) std::memcpy(S, s.data(), sizeof(T) * s.size());
         return K;
     });
 }
@@ -928,7 +945,7 @@ long tiledC(int alg, const T* P, long n, int dim, T r1, T r2, long cap, const lo
         std::vector<T> out;
         long ns = oracle::tiledRD<T>(alg, P, n, dim, r1, r2, (size_t)cap, t, refinement != 0, maxEvolveIter, out);
         if (ns > Scap) throw std::runtime_error("tiled: output capacity too small");
-        std::memcpy(S, out.data(), sizeof(T) * out.size());
+        if (!out.empty()) std::memcpy(S, out.data(), sizeof(T) * out.size());
         return ns;
     });
 }
@@ -1112,6 +1129,10 @@ int oracle_num_threads() {
     long oracle_rd_##SUF(const T* P, long n, int dim, T r1, T r2, long maxEvolveIter, T* S,          \
                          long* rounds) {                                                             \
         return rdC<T>(P, n, dim, r1, r2, maxEvolveIter, S, rounds);                                  \
+    }                                                                                                \
+    long oracle_rd_own_##SUF(const T* P, long n, long nOwn, int dim, T r1, T r2, long maxEvolveIter,  \
+                             T* S, long* rounds) {                                                   \
+        return rdOwnC<T>(P, n, nOwn, dim, r1, r2, maxEvolveIter, S, rounds);                         \
     }                                                                                                \
     long oracle_tiled_##SUF(int alg, const T* P, long n, int dim, T r1, T r2, long cap,              \
                             const long* tpd, int refinement, long maxEvolveIter, T* S, long Scap) {  \

@@ -1,10 +1,10 @@
 // ref_driver.cpp -- C ABI over the reference's own header-only CPU implementation.
 //
 // TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the reference headers
-// where they lie (`/root/reference/ridge_detection`), into oracle/_ref/.  It is used to
-// pin the restatement in rd_oracle.cpp (tests/test_oracle_vs_ref.py), to write the golden
-// fixtures under tests/golden/ (make_golden.py), and -- in its OpenMP build -- as the timed
-// CPU baseline of bench.py.  Nothing in the product loads it.
+// where they lie (`/root/reference/ridge_detection`), into oracle/_ref/.  It writes the golden
+// fixtures under tests/golden/ (make_golden.py), which pin the restatement in rd_oracle.cpp
+// (tests/test_oracle_golden.py), and -- in its OpenMP build -- it is the timed CPU baseline
+// of bench.py.  Nothing in the product loads it.
 //
 // The reference copies coordinates rather than indices; indices are recovered here by
 // matching coordinates back to the input (inputs used with this driver have no

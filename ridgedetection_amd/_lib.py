@@ -33,11 +33,12 @@ EXPORTS = [
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
     "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats", "rdhip_batch_ridge_detection_dims",
     "rdhip_generate", "rdhip_generate_points", "rdhip_batch_generate", "rdhip_batch_run",
+    "rdhip_set_chosen_ex", "rdhip_ridge_detection_own",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
 OPT_LOCAL_MAX_SAMPLES, OPT_LOCAL_MAX_WORK, OPT_SCAN_THREADS, OPT_AUTOTUNE, OPT_SEGMENTED = 6, 7, 8, 9, 10
-OPT_POSITION_COPY = 11
+OPT_POSITION_COPY, OPT_EVOLVE_GUARD, OPT_INJECT_ALLOC_FAILURE = 11, 12, 13
 GEN_CIRCLE, GEN_SPHERE, GEN_SPIRAL2D, GEN_SPIRAL3D, GEN_SEGMENT, GEN_HELIX, GEN_POLYLINE, GEN_UNIFORM = range(8)
 
 
@@ -75,6 +76,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_get_timers": (i32, [vp, C.POINTER(Timers)]),
         "rdhip_choose": (i32, [vp, C.c_double, C.POINTER(i64), vp]),
         "rdhip_set_chosen": (i32, [vp, vp, i64, i32]),
+        "rdhip_set_chosen_ex": (i32, [vp, vp, i64, i32, i64, i32]),
+        "rdhip_ridge_detection_own": (i32, [vp, i64, C.c_double, C.c_double, i64, C.POINTER(i64)]),
         "rdhip_evolve": (i32, [vp, C.c_double, i64, C.POINTER(i64)]),
         "rdhip_evolve_iteration": (i32, [vp, C.c_double, vp, vp, vp, vp, C.POINTER(i32)]),
         "rdhip_decimate": (i32, [vp, C.c_double, i32, vp, C.POINTER(i64)]),
@@ -196,6 +199,13 @@ class Context:
                                                    C.byref(k)))
         return self.points()
 
+    def ridge_detection_own(self, n_own, r1, r2, max_evolve_iters=0):
+        """Ridge detection choosing from the first n_own points, evolving over all of them."""
+        k = C.c_int64()
+        self._check(self.lib.rdhip_ridge_detection_own(self.h, int(n_own), float(r1), float(r2),
+                                                       int(max_evolve_iters), C.byref(k)))
+        return self.points()
+
     def batch(self, clouds, r1, r2, max_evolve_iters=0):
         """Batched ridge detection over clouds (list of (n_c, dim) arrays) with per-cloud radii.
         Returns the list of per-cloud ridge points."""
@@ -299,9 +309,14 @@ class Context:
         self._check(self.lib.rdhip_choose(self.h, float(r), C.byref(k), _ptr(idx)))
         return self.points(), idx[:k.value].copy()
 
-    def set_chosen(self, S):
+    def set_chosen(self, S, layout=ROW_MAJOR, stride=0):
+        """Chosen points from host memory: (K, dim) ROW_MAJOR, or (dim, K) COL_MAJOR / SOA."""
         S = np.ascontiguousarray(S, dtype=self.dtype)
-        self._check(self.lib.rdhip_set_chosen(self.h, _ptr(S), S.shape[0], ROW_MAJOR))
+        K = S.shape[0] if layout == ROW_MAJOR else S.shape[-1]
+        if layout == ROW_MAJOR and stride == 0:
+            self._check(self.lib.rdhip_set_chosen(self.h, _ptr(S), K, ROW_MAJOR))
+        else:
+            self._check(self.lib.rdhip_set_chosen_ex(self.h, _ptr(S), K, layout, stride, 0))
 
     def evolve(self, r, max_iters=0):
         it = C.c_int64()

@@ -30,6 +30,7 @@ struct rdhip_ctx {
     std::unique_ptr<TileTreeBuilder<double>> tree64;
     int treeKind = -1;
     std::vector<int64_t> batchSizes;   // clouds of the resident batch (rdhip_batch_generate)
+    bool broken = false;               // a HIP error left device state undefined
 
     std::unique_ptr<TileTreeBuilder<float>>& tree(float) { return tree32; }
     std::unique_ptr<TileTreeBuilder<double>>& tree(double) { return tree64; }
@@ -61,14 +62,20 @@ int guarded(F&& f) {
     }
 }
 
-// Runs `body(engine)` on the context's engine of the right value type.
+// Runs `body(engine)` on the context's engine of the right value type.  After a HIP error
+// (failed allocation, launch or copy) the context's device buffers may be half set up, so
+// every later call on it fails until it is destroyed.
 template <typename F>
 int withEngine(rdhip_ctx* ctx, F&& body) {
     if (!ctx) return fail(RDHIP_ERR_INVALID, "null context");
-    return guarded([&]() -> int {
+    if (ctx->broken)
+        return fail(RDHIP_ERR_STATE, "an earlier HIP error left this context unusable; destroy it and create another");
+    const int rc = guarded([&]() -> int {
         if (ctx->dtype == RDHIP_F32) return body(*ctx->f32, (float)0);
         return body(*ctx->f64, (double)0);
     });
+    if (rc == RDHIP_ERR_HIP) ctx->broken = true;
+    return rc;
 }
 
 }  // namespace
@@ -136,13 +143,20 @@ int rdhip_set_points(rdhip_ctx* ctx, const void* points, int64_t n, int layout, 
 }
 
 int rdhip_ridge_detection(rdhip_ctx* ctx, double r1, double r2, int64_t max_evolve_iters, int64_t* out_count) {
+    return rdhip_ridge_detection_own(ctx, -1, r1, r2, max_evolve_iters, out_count);
+}
+
+int rdhip_ridge_detection_own(rdhip_ctx* ctx, int64_t n_own, double r1, double r2, int64_t max_evolve_iters,
+                              int64_t* out_count) {
     if (!(r1 >= 0) || !(r2 >= 0)) return fail(RDHIP_ERR_INVALID, "radii must be non-negative");
     if (max_evolve_iters < 0) return fail(RDHIP_ERR_INVALID, "negative iteration limit");
     return withEngine(ctx, [&](auto& e, auto zero) -> int {
         using T = decltype(zero);
         if (!ctx->hasPoints || e.numPoints() < 1) return fail(RDHIP_ERR_STATE, "no input points");
+        if (n_own == 0 || n_own > e.numPoints())
+            return fail(RDHIP_ERR_INVALID, "own sample count must be in [1, number of points] (or negative for all)");
         e.timers = Timers();
-        e.setSingleSegment((T)r1, (T)r2);
+        e.setSingleSegment((T)r1, (T)r2, n_own);
         RunOptions opt = e.options;
         opt.maxEvolveIters = max_evolve_iters;
         e.ridgeDetectionLocal(opt);
@@ -560,6 +574,11 @@ int rdhip_get_tree(rdhip_ctx* ctx, int64_t* leaf_info, int32_t* own_idx, int32_t
 }
 
 int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
+    if (ctx && option == RDHIP_OPT_INJECT_ALLOC_FAILURE) {   // no device state: also on a broken context
+        if (value < 0) return fail(RDHIP_ERR_INVALID, "negative allocation count");
+        t_allocFailCountdown = value;
+        return RDHIP_OK;
+    }
     return withEngine(ctx, [&](auto& e, auto) -> int {
         switch (option) {
             case RDHIP_OPT_PPT:
@@ -612,6 +631,10 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
                 if (value < 0) return fail(RDHIP_ERR_INVALID, "negative local work limit");
                 e.options.localMaxWork = value;
                 return RDHIP_OK;
+            case RDHIP_OPT_EVOLVE_GUARD:
+                if (value < 0) return fail(RDHIP_ERR_INVALID, "negative iteration guard");
+                e.options.evolveGuard = value;
+                return RDHIP_OK;
         }
         return fail(RDHIP_ERR_INVALID, "unknown option");
     });
@@ -648,15 +671,24 @@ int rdhip_choose(rdhip_ctx* ctx, double r, int64_t* out_count, int32_t* out_indi
 }
 
 int rdhip_set_chosen(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout) {
+    return rdhip_set_chosen_ex(ctx, chosen, count, layout, 0, 0);
+}
+
+int rdhip_set_chosen_ex(rdhip_ctx* ctx, const void* chosen, int64_t count, int layout, int64_t stride, int on_device) {
     if (count < 0) return fail(RDHIP_ERR_INVALID, "negative count");
     if (count > 0 && !chosen) return fail(RDHIP_ERR_INVALID, "null chosen points");
-    if (layout != RDHIP_ROW_MAJOR && layout != RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "layout must be ROW_MAJOR or SOA");
+    if (layout < RDHIP_ROW_MAJOR || layout > RDHIP_SOA) return fail(RDHIP_ERR_INVALID, "unknown layout");
+    if (stride < 0) return fail(RDHIP_ERR_INVALID, "negative stride");
     return withEngine(ctx, [&](auto& e, auto zero) -> int {
         using T = decltype(zero);
         if (!ctx->hasPoints) return fail(RDHIP_ERR_STATE, "no input points");
-        if (count > e.numPoints() + 4096) return fail(RDHIP_ERR_INVALID, "more chosen points than capacity");
+        if (count > e.numPoints()) return fail(RDHIP_ERR_INVALID, "more chosen points than input points");
+        if (layout == RDHIP_ROW_MAJOR && stride != 0 && stride < ctx->dim)
+            return fail(RDHIP_ERR_INVALID, "row stride smaller than the dimension");
+        if (layout != RDHIP_ROW_MAJOR && stride != 0 && stride < count)
+            return fail(RDHIP_ERR_INVALID, "column stride smaller than the point count");
         e.setSingleSegment(T(0), T(0));
-        e.setChosen(0, static_cast<const T*>(chosen), count, layout);
+        e.setChosen(0, static_cast<const T*>(chosen), count, layout, stride, on_device != 0);
         ctx->hasChosen = true;
         return RDHIP_OK;
     });

@@ -21,6 +21,19 @@
 
 namespace rdhip {
 
+// Fault injection for tests (RDHIP_OPT_INJECT_ALLOC_FAILURE): when positive, the device
+// allocation that brings this count to zero fails with hipErrorOutOfMemory.  Per thread.
+inline thread_local long long t_allocFailCountdown = 0;
+
+// Every device allocation of the library goes through here.
+inline hipError_t deviceMalloc(void** p, size_t bytes) {
+    if (t_allocFailCountdown > 0 && --t_allocFailCountdown == 0) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
 constexpr int kWave = 64;
 constexpr int kChosenTile = 16;   // chosen points per screening tile of the evolve scan
 constexpr int kMaxDim = 12;

@@ -555,8 +555,9 @@ hipError_t dec_ordered_removal(DecArgs& a, int64_t maxElems, DecWorkspace& ws, h
     if (need > ws.parCap) {
         if (ws.par) (void)hipFree(ws.par);
         ws.par = nullptr;
+        ws.parCap = 0;
+        if ((err = deviceMalloc((void**)&ws.par, sizeof(int) * (need + need / 4))) != hipSuccess) return err;
         ws.parCap = need + need / 4;
-        if ((err = hipMalloc(&ws.par, sizeof(int) * ws.parCap)) != hipSuccess) return err;
     }
     int* p = ws.par;
     const int64_t me = std::max<int64_t>(maxElems, 1);
@@ -633,8 +634,9 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     if (boxBytes > ws.boxCap) {
         if (ws.boxes) (void)hipFree(ws.boxes);
         ws.boxes = nullptr;
+        ws.boxCap = 0;
+        if ((err = deviceMalloc((void**)&ws.boxes, boxBytes + boxBytes / 4)) != hipSuccess) return err;
         ws.boxCap = boxBytes + boxBytes / 4;
-        if ((err = hipMalloc(&ws.boxes, ws.boxCap)) != hipSuccess) return err;
     }
     T* boxes = static_cast<T*>(ws.boxes);
     T* coords = boxes + (maxChunks + maxSupers) * 2 * dim;
@@ -661,8 +663,11 @@ hipError_t decimate_groups(DecArgs& a, T* S, int64_t sStride, int dim, const T* 
     if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
     if (nbrTotal > ws.nbrCap) {
         if (ws.nbr) (void)hipFree(ws.nbr);
+        ws.nbr = nullptr;
+        ws.nbrCap = 0;
+        if ((err = deviceMalloc((void**)&ws.nbr, sizeof(int) * (nbrTotal + nbrTotal / 4 + 1024))) != hipSuccess)
+            return err;
         ws.nbrCap = nbrTotal + nbrTotal / 4 + 1024;
-        if ((err = hipMalloc(&ws.nbr, sizeof(int) * ws.nbrCap)) != hipSuccess) return err;
     }
     a.nbr = ws.nbr;
     if (prov)

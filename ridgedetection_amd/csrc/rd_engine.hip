@@ -21,10 +21,6 @@ namespace {
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-// An uncapped evolve that has not converged after this many iterations is reported as an
-// error instead of looping forever.
-constexpr long long kEvolveGuard = 100000;
-
 __global__ void fill_int_kernel(int* __restrict__ a, int64_t n, int v, const int* __restrict__ skip) {
     if (skip && *skip) return;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a[i] = v;
@@ -176,13 +172,14 @@ void Engine<T>::setPoints(const T* src, int64_t n, int layout, int64_t stride, b
 }
 
 template <typename T>
-void Engine<T>::setSingleSegment(T r1, T r2) {
+void Engine<T>::setSingleSegment(T r1, T r2, int64_t nOwn) {
+    if (nOwn < 0 || nOwn > nPoints_) nOwn = nPoints_;
     SegDesc s{};
     s.pOff = 0;
     s.nP = (int32_t)nPoints_;
-    s.nOwn = (int32_t)nPoints_;
+    s.nOwn = (int32_t)nOwn;
     s.group = 0;
-    setSegments({s}, {std::max<int64_t>(nPoints_, 1)}, {r1}, {r2}, false);
+    setSegments({s}, {std::max<int64_t>(nOwn, 1)}, {r1}, {r2}, false);
 }
 
 template <typename T>
@@ -891,13 +888,14 @@ int Engine<T>::evolveLocal(long long maxIter, std::vector<int>& done) {
     LocalArgs<T> a{es.P, es.stride, es.index, S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegR1Sqr_.get(),
                    dLocalList_.get(), (int)list.size(), (int)round_up(std::max(maxK, 1), kChosenTile),
                    std::min(std::max(maxP, 1), kLocalChunk), maxIter > 0 ? (int)std::min<long long>(maxIter, 0x7fffffff) : 0,
-                   (int)kEvolveGuard, dSegIter_.get(), flags_.get() + 5, flags_.get() + 6,
+                   options.evolveGuard > 0 ? (int)std::min<long long>(options.evolveGuard, 0x7fffffff) : 0x7fffffff,
+                   dSegIter_.get(), flags_.get() + 5, flags_.get() + 6,
                    nullptr, nullptr, nullptr, nullptr};
     check(launch_local_evolve<T>(a, dim_, st_), "local_evolve");
     int res[2] = {0, 0};
     check(hipMemcpyAsync(res, flags_.get() + 5, sizeof(int) * 2, hipMemcpyDeviceToHost, st_), "local flags");
     check(hipStreamSynchronize(st_), "local_evolve");
-    if (res[1]) throw std::runtime_error("evolve did not converge within the iteration guard");
+    if (res[1]) throw std::runtime_error(guardMessage());
     for (int s : list) done[s] = 1;
     return res[0];
 }
@@ -964,9 +962,9 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         if (options.profile) readEvents(batch);
         iters = f[1];
         if (f[0]) break;
-        if (iters >= kEvolveGuard) {
+        if (options.evolveGuard > 0 && iters >= options.evolveGuard) {
             if (exec) (void)hipGraphExecDestroy(exec);
-            throw std::runtime_error("evolve did not converge within the iteration guard");
+            throw std::runtime_error(guardMessage());
         }
         if (compact() && !segmentedNow_) {
             // shrink the active work list once half of it belongs to converged segments
@@ -1328,19 +1326,33 @@ std::vector<int> Engine<T>::chosenIndicesHost(int seg) {
 }
 
 template <typename T>
-void Engine<T>::setChosen(int seg, const T* S, int64_t K, int layout) {
+void Engine<T>::setChosen(int seg, const T* S, int64_t K, int layout, int64_t stride, bool onDevice) {
     ++version_;
     chosenFromPoints_ = false;
     if (seg < 0 || seg >= (int)segs_.size()) throw std::invalid_argument("segment out of range");
     if (K < 0 || K > segs_[seg].sCap) throw std::invalid_argument("too many chosen points for the segment");
-    std::vector<T> soa((size_t)K * dim_);
-    for (int64_t k = 0; k < K; ++k)
-        for (int d = 0; d < dim_; ++d)
-            soa[(size_t)d * K + k] = layout == LAYOUT_ROW_MAJOR ? S[k * dim_ + d] : S[(size_t)d * K + k];
-    if (K > 0)
-        check(hipMemcpy2DAsync(S_.get() + segs_[seg].sOff, sizeof(T) * sStride_, soa.data(), sizeof(T) * K,
-                               sizeof(T) * K, dim_, hipMemcpyHostToDevice, st_),
+    if (layout != LAYOUT_ROW_MAJOR && layout != LAYOUT_COL_MAJOR && layout != LAYOUT_SOA)
+        throw std::invalid_argument("unsupported memory layout");
+    const bool row = layout == LAYOUT_ROW_MAJOR;
+    if (stride <= 0) stride = row ? dim_ : K;
+    T* dst = S_.get() + segs_[seg].sOff;
+    if (K > 0 && onDevice && row) {
+        hipLaunchKernelGGL((row_to_soa_kernel<T>), dim3(grid_for(K)), dim3(256), 0, st_, S, K, dim_, stride, dst,
+                           sStride_);
+        check(hipGetLastError(), "row_to_soa");
+    } else if (K > 0 && onDevice) {
+        check(hipMemcpy2DAsync(dst, sizeof(T) * sStride_, S, sizeof(T) * stride, sizeof(T) * K, dim_,
+                               hipMemcpyDeviceToDevice, st_),
               "setChosen");
+    } else if (K > 0) {
+        std::vector<T> soa((size_t)K * dim_);
+        for (int64_t k = 0; k < K; ++k)
+            for (int d = 0; d < dim_; ++d)
+                soa[(size_t)d * K + k] = row ? S[k * stride + d] : S[(size_t)d * stride + k];
+        check(hipMemcpy2DAsync(dst, sizeof(T) * sStride_, soa.data(), sizeof(T) * K, sizeof(T) * K, dim_,
+                               hipMemcpyHostToDevice, st_),
+              "setChosen");
+    }
     int k32 = (int)K;
     check(hipMemcpyAsync(dSegK_.get() + seg, &k32, sizeof(int), hipMemcpyHostToDevice, st_), "K");
     std::vector<int> idx((size_t)std::max<int64_t>(K, 1));

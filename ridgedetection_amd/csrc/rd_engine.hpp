@@ -45,7 +45,7 @@ struct DevBuf {
         if (count <= n && p) return p;
         release();
         size_t c = count ? count : 1;
-        check(hipMalloc(&p, sizeof(T) * c), "hipMalloc");
+        check(deviceMalloc((void**)&p, sizeof(T) * c), "hipMalloc");
         n = c;
         return p;
     }
@@ -91,6 +91,8 @@ struct RunOptions {
     bool segmented = false;         // batches whose active segments all have <= segmentedMaxP
     int segmentedMaxP = 65536;      // samples: batch-wide scan + per-segment sort and sums
     bool positionCopy = true;       // indexed segments: evolve reads a position-ordered copy
+    long long evolveGuard = 100000; // an evolve still moving after this many iterations fails
+                                    // (0 = no limit, as in the reference)
 };
 
 template <typename T>
@@ -121,7 +123,8 @@ public:
     void setSegments(const std::vector<SegDesc>& segs, const std::vector<int64_t>& capacities,
                      const std::vector<T>& r1, const std::vector<T>& r2, bool indexed = false);
     int* reserveIndex(size_t positions) { return pIndexBuf_.reserve(positions); }
-    void setSingleSegment(T r1, T r2);
+    // one segment over all points; choose takes the first nOwn of them (< 0: all)
+    void setSingleSegment(T r1, T r2, int64_t nOwn = -1);
     // Replaces the radii of the current segments without touching their chosen points.
     void setRadii(const std::vector<T>& r1, const std::vector<T>& r2);
     int numSegments() const { return (int)segs_.size(); }
@@ -181,10 +184,16 @@ public:
     // Chosen points of all segments in segment order, into `out` with `layout`.
     void getChosen(T* out, int layout, bool onDevice, int64_t outStride = 0);
     std::vector<int> chosenIndicesHost(int seg);
-    void setChosen(int seg, const T* S, int64_t K, int layout);   // host ROW_MAJOR/SOA input
+    // chosen points of segment `seg` from host or device memory: ROW_MAJOR (stride >= dim
+    // between points) or COL_MAJOR / SOA (stride >= K between coordinates); 0 = packed
+    void setChosen(int seg, const T* S, int64_t K, int layout, int64_t stride = 0, bool onDevice = false);
 
     Timers timers;
     RunOptions options;
+    std::string guardMessage() const {
+        return "evolve did not converge within the iteration guard (" + std::to_string(options.evolveGuard) +
+               " iterations; RDHIP_OPT_EVOLVE_GUARD raises it, 0 removes it)";
+    }
     int64_t lastEvolveIters = 0;
 
 private:

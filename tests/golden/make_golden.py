@@ -10,6 +10,8 @@ not depend on libm.  Run: python tests/golden/make_golden.py
 from __future__ import annotations
 
 import os
+import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -74,7 +76,56 @@ def tiled_case(ref, name, P, r1, r2, tpd, cap):
         f"{a}:{len(d[f'alg{a}_ref0_S'])}" for a in (0, 1, 2, 4, 5)))
 
 
+SAMPLES_IO = os.path.join(ROOT, "oracle", "_ref", "ref_samples_io")
+REF_DIR = os.path.dirname(SAMPLES_IO)
+
+
+def _samples_io(*args):
+    r = subprocess.run([SAMPLES_IO, *[str(a) for a in args]], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_samples_io {args}: {r.returncode} {r.stderr}")
+
+
+def _read_loaded(name):
+    with open(os.path.join(REF_DIR, name + ".bin"), "rb") as f:
+        n, dim = np.frombuffer(f.read(16), dtype=np.int64)
+        return np.frombuffer(f.read(), dtype=np.float64).reshape(int(n), int(dim))
+
+
+def samples_case():
+    """Text sample files written by the reference's Samples<T>::saveToFile (float: fixed, 4
+    decimals, width 10; double: 12 decimals, width 16) and the values its readers return."""
+    g = np.random.Generator(np.random.PCG64(2024))
+    base = synth.helix3d(37, seed=21, dtype=np.float64) * 7.0
+    special = np.array([[0.0, -0.0, 2.5e-5], [-2.5e-5, 1.00005, -1.00005], [12345678.9, -1e-9, 1e10],
+                        [0.123456789012345, -987.654321098765, 3.0]], dtype=np.float64)
+    P64 = np.concatenate([base, special, g.normal(0.0, 100.0, (9, 3))])
+    d = {}
+    # the descriptor as the reference's generator programs write it
+    # (tests/test_gpu_samples_set/samples_generator.cu:56): leading space, "--" keys, a
+    # trailing newline (so a blank line follows it in the file)
+    desc = f" --dim=3 --np={len(P64)} --a=31.4159 --b=0.15  --s=0.05\n"
+    for dt, isd, sfx in ((np.float32, 0, "f32"), (np.float64, 1, "f64")):
+        P = P64.astype(dt)
+        blob = os.path.join(REF_DIR, f"in_{sfx}.bin")
+        with open(blob, "wb") as f:
+            f.write(np.array([len(P), 3, isd], dtype=np.int64).tobytes() + P.tobytes())
+        for name, comment in ((f"samples_{sfx}", desc), (f"samples_plain_{sfx}", "")):
+            _samples_io("save", blob, name, *([comment] if comment else []))
+            shutil.copy(os.path.join(REF_DIR, name + ".txt"), os.path.join(OUT, name + ".txt"))
+        _samples_io("loadgen", f"samples_{sfx}.txt", isd)
+        d[f"loadgen_{sfx}"] = _read_loaded(f"samples_{sfx}.txt").astype(dt)
+        _samples_io("loadgen", f"samples_{sfx}.txt", isd, 2)
+        d[f"loadgen_dim2_{sfx}"] = _read_loaded(f"samples_{sfx}.txt").astype(dt)
+        _samples_io("load", f"samples_plain_{sfx}.txt", isd)
+        d[f"load_plain_{sfx}"] = _read_loaded(f"samples_plain_{sfx}.txt").astype(dt)
+        d[f"P_{sfx}"] = P
+    np.savez_compressed(os.path.join(OUT, "samples_io.npz"), **d)
+    print(f"samples: {len(P64)} points, text files for f32 and f64")
+
+
 def main():
+    samples_case()
     ref = Oracle("ref")
     for dt, sfx in ((np.float32, "f32"), (np.float64, "f64")):
         brute_case(ref, f"spiral20k_{sfx}", synth.spiral2d(20000, seed=1, dtype=dt), 0.45, 0.9)
