@@ -325,6 +325,25 @@ int rdhip_evolve_fwd(rdhip_ctx* ctx, double r, int64_t iters, int64_t checkpoint
 int rdhip_evolve_bwd(rdhip_ctx* ctx, const void* grad_chosen, void* grad_points, void* grad_seeds, int layout,
                      int on_device, int seed_to_points);
 
+/* ---- the choose without allocation -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the greedy
+ * choose of rdhip_choose (rd/cpu/brute_force/choose.hpp:36-70) on caller-owned device buffers,
+ * with no context.  It allocates nothing, copies nothing and does not synchronise: it only
+ * enqueues kernels on hip_stream (NULL = the null stream), which must belong to the current
+ * device.  Its work memory (about 16 MiB) is a fixed device array of the library, one per
+ * device, held until the process ends; calls that overlap on one device race.
+ *   points:  n points (device), COL_MAJOR / SOA: coordinate d of point i at points[d*stride + i]
+ *            (stride >= n, 0 = n); n in [1, RDHIP_NOALLOC_MAX_POINTS];
+ *   chosen:  device, capacity n per coordinate: chosen point k at chosen[d*chosen_stride + k]
+ *            (chosen_stride >= n, 0 = n);
+ *   indices: device int32[n]: the chosen points' input indices, ascending;
+ *   count:   device int32: the number of chosen points.
+ * Results equal rdhip_choose's. */
+#define RDHIP_NOALLOC_MAX_POINTS (1 << 24)
+int rdhip_choose_noalloc(int dtype, int dim, const void* points, int64_t n, int64_t stride, double r, void* chosen,
+                         int64_t chosen_stride, int32_t* indices, int32_t* count, void* hip_stream);
+
 /* Decimate the chosen set with radius r2 and `rule`; keep (host, optional, capacity K)
  * receives 1 for kept and 0 for removed points, in the pre-decimation order. */
 int rdhip_decimate(rdhip_ctx* ctx, double r2, int rule, uint8_t* keep, int64_t* out_count);

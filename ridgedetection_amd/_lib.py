@@ -33,7 +33,7 @@ EXPORTS = [
     "rdhip_batch_ridge_detection", "rdhip_evolve_fwd", "rdhip_evolve_bwd", "rdhip_order_chains",
     "rdhip_set_distances", "rdhip_hausdorff", "rdhip_distance_stats", "rdhip_batch_ridge_detection_dims",
     "rdhip_generate", "rdhip_generate_points", "rdhip_batch_generate", "rdhip_batch_run",
-    "rdhip_set_chosen_ex", "rdhip_ridge_detection_own",
+    "rdhip_set_chosen_ex", "rdhip_ridge_detection_own", "rdhip_choose_noalloc",
 ]
 TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
@@ -78,6 +78,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "rdhip_set_chosen": (i32, [vp, vp, i64, i32]),
         "rdhip_set_chosen_ex": (i32, [vp, vp, i64, i32, i64, i32]),
         "rdhip_ridge_detection_own": (i32, [vp, i64, C.c_double, C.c_double, i64, C.POINTER(i64)]),
+        "rdhip_choose_noalloc": (i32, [i32, i32, vp, i64, i64, C.c_double, vp, i64, vp, vp, vp]),
         "rdhip_evolve": (i32, [vp, C.c_double, i64, C.POINTER(i64)]),
         "rdhip_evolve_iteration": (i32, [vp, C.c_double, vp, vp, vp, vp, C.POINTER(i32)]),
         "rdhip_decimate": (i32, [vp, C.c_double, i32, vp, C.POINTER(i64)]),

@@ -80,6 +80,10 @@ int withEngine(rdhip_ctx* ctx, F&& body) {
 
 }  // namespace
 
+namespace rdhip {
+void set_last_error(const char* msg) { g_error = msg; }
+}  // namespace rdhip
+
 extern "C" {
 
 int rdhip_version(void) { return 100; }

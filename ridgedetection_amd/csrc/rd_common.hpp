@@ -34,6 +34,9 @@ inline hipError_t deviceMalloc(void** p, size_t bytes) {
     return hipMalloc(p, bytes);
 }
 
+// The message rdhip_last_error() returns on the calling thread (defined in rd_api.hip).
+void set_last_error(const char* msg);
+
 constexpr int kWave = 64;
 constexpr int kChosenTile = 16;   // chosen points per screening tile of the evolve scan
 constexpr int kMaxDim = 12;

@@ -41,3 +41,14 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert lib.rdhip_set_points(None, None, -5, 0, 0, 0) == -1
     assert lib.rdhip_destroy(None) == 0
     assert lib.rdhip_version() >= 100
+
+
+def test_noalloc_choose_validates_before_touching_the_device():
+    from ridgedetection_amd import _lib
+    lib = _lib.load()
+    fake = C.c_void_p(256)   # never dereferenced: each call fails validation first
+    assert lib.rdhip_choose_noalloc(0, 3, fake, (1 << 24) + 1, 0, 0.1, fake, 0, fake, fake, None) == -1
+    assert b"RDHIP_NOALLOC_MAX_POINTS" in lib.rdhip_last_error()
+    assert lib.rdhip_choose_noalloc(0, 3, fake, 0, 0, 0.1, fake, 0, fake, fake, None) == -1
+    assert lib.rdhip_choose_noalloc(5, 3, fake, 10, 0, 0.1, fake, 0, fake, fake, None) == -1
+    assert lib.rdhip_choose_noalloc(0, 3, fake, 10, 0, 0.1, fake, 9, fake, fake, None) == -1
