@@ -41,6 +41,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """Host cores this process may use and the CPU model (the CPU baseline's machine)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": cores, "cpu_model": model}
+
+
 def cpu_baseline(P, S, r, budget_s=24.0):
     """Times the reference's OpenMP evolve pass (calculateClosestSphereCenter_omp +
     shiftTowardMassCenter_omp) over the same cloud and chosen set."""
@@ -66,15 +84,77 @@ def cpu_baseline(P, S, r, budget_s=24.0):
     value = reps * m * k / el
     return {"value": value, "unit": "pair-evals/s", "cores": threads, "kind": kind,
             "sample": f"{reps} evolve pass(es) of {m} of the {n} points x all {k} chosen points, "
-                      f"{el:.1f} s on {threads} host threads"}
+                      f"{el:.1f} s on {threads} host threads", "host": host_info()}
 
 
-def batched_vs_looped(count=512, loop_every=8, seed=7, graph=True, batch_only=False):
+def cpu_per_config():
+    """Whole ridge detection on the host (the reference's OpenMP build, all host threads) next
+    to the GPU on the same inputs: the 20k-point spiral (r 0.45), a 50k-point sample of the 1M
+    helix (its first 50k points, r 0.2) with the GPU also on the whole cloud, and the helix
+    tilings (8 x 8 x 8 tiles split at 10 000 points) with the reference's own (whole, build,
+    rdTiles, refinement) times."""
+    from oracle.oracle import Oracle, available
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context
+    if not available("ref_omp"):
+        return None
+    o = Oracle("ref_omp")
+    threads = int(o.lib.ref_openmp_threads())
+    out = {"kind": "reference", "threads": threads, "host": host_info()}
+
+    def gpu_rd(P, r1, r2, cap=0):
+        with Context(P.shape[1], P.dtype) as c:
+            c.set_points(P)
+            c.ridge_detection(r1, r2, max_evolve_iters=cap)      # warm (kernels, allocations)
+            c.set_points(P)
+            t0 = time.perf_counter()
+            S = c.ridge_detection(r1, r2, max_evolve_iters=cap)
+            return time.perf_counter() - t0, len(S)
+
+    P = synth.spiral2d(20000, seed=1)
+    t0 = time.perf_counter()
+    S, _ = o.ridge_detection(P, 0.45, 0.9, omp=True)
+    cpu = time.perf_counter() - t0
+    gpu, k = gpu_rd(P, 0.45, 0.9, cap=100)
+    out["spiral20k_brute"] = {"cpu_s": cpu, "gpu_s": gpu, "cpu_ridge_points": len(S), "gpu_ridge_points": k,
+                              "gpu_over_cpu_speedup": cpu / gpu}
+    log(f"bench: CPU spiral20k whole RD {cpu:.2f} s, GPU {gpu * 1e3:.1f} ms")
+    H = synth.helix3d(1_000_000, seed=2024, sigma=0.05, dtype=np.float32)
+    sub = np.ascontiguousarray(H[:50000])
+    t0 = time.perf_counter()
+    S, _ = o.ridge_detection(sub, 0.2, 0.4, omp=True)
+    cpu = time.perf_counter() - t0
+    gpu, k = gpu_rd(sub, 0.2, 0.4)
+    gfull, kfull = gpu_rd(H, 0.2, 0.4)
+    out["helix_brute"] = {"sample": "first 50 000 of the 1M helix points", "cpu_s": cpu, "gpu_s": gpu,
+                          "cpu_ridge_points": len(S), "gpu_ridge_points": k, "gpu_over_cpu_speedup": cpu / gpu,
+                          "gpu_whole_1M_s": gfull, "gpu_whole_1M_ridge_points": kfull}
+    log(f"bench: CPU helix 50k whole RD {cpu:.2f} s, GPU {gpu * 1e3:.1f} ms (1M: {gfull:.2f} s)")
+    for alg in (0, 5):
+        t0 = time.perf_counter()
+        S = o.tiled(alg, H, 0.2, 0.4, 10000, [8, 8, 8], threads=threads)
+        cpu = time.perf_counter() - t0
+        whole, build, tiles, refine = (float(x) for x in o.last_times_ms)
+        with Context(3, np.float32) as c:
+            c.set_points(H)
+            c.tiled(alg, 0.2, 0.4, 10000, [8, 8, 8])                # warm
+            t0 = time.perf_counter()
+            G = c.tiled(alg, 0.2, 0.4, 10000, [8, 8, 8])
+            gpu = time.perf_counter() - t0
+        out[f"helix_tiled/{TILED_NAMES[alg]}"] = {
+            "cpu_s": cpu, "cpu_whole_ms": whole, "cpu_build_ms": build, "cpu_rd_tiles_ms": tiles,
+            "cpu_refinement_ms": refine, "gpu_s": gpu, "cpu_ridge_points": len(S), "gpu_ridge_points": len(G),
+            "gpu_over_cpu_speedup": cpu / gpu}
+        log(f"bench: CPU helix tiled {alg}: {cpu:.2f} s, GPU {gpu * 1e3:.1f} ms")
+    return out
+
+
+def batched_vs_looped(count=512, loop_every=1, seed=7, graph=True, batch_only=False):
     """512 clouds (256 with d=2, 256 with d=3), sizes log-uniform in [2000, 500000], each
     a noisy spiral / helix / segment chain with r1 = 4 sigma, r2 = 2 r1: one batch of all
     clouds (one launch per phase; the 2-D clouds run zero-padded to 3 coordinates, which
-    leaves their results bit-identical) against a per-cloud loop over every
-    `loop_every`-th cloud, each in a context of its own dimension.  Both forms start from
+    leaves their results bit-identical) against a per-cloud loop over every cloud (every
+    `loop_every`-th with loop_every > 1), each in a context of its own dimension.  Both forms start from
     host arrays (uploads included).  The batch split by dimension (two batches, one per
     context dimension) is timed too."""
     from ridgedetection_amd import synth
@@ -104,11 +184,15 @@ def batched_vs_looped(count=512, loop_every=8, seed=7, graph=True, batch_only=Fa
     t0 = time.perf_counter()
     out_counts = run(ctxs[dmax], list(range(count)))
     t_batch = time.perf_counter() - t0
+    tb = ctxs[dmax].timers()
+    shape = {"samples_per_lane": tb["scan_ppt"], "threads_per_block": tb["scan_threads"],
+             "persistent": bool(tb["scan_persistent"]), "tune_ms": tb["tune_ms"]}
     log(f"bench: batched {count} clouds in {t_batch:.2f} s")
     if batch_only:
         for c in ctxs.values():
             c.close()
-        return {"clouds": count, "points": total_pts, "ridge_points": out_counts, "batched_s": t_batch}
+        return {"clouds": count, "points": total_pts, "ridge_points": out_counts, "batched_s": t_batch,
+                "scan_shape": shape}
     t0 = time.perf_counter()
     split_counts = sum(run(ctxs[d], idx) for d, idx in by_dim.items())
     t_split = time.perf_counter() - t0
@@ -135,8 +219,9 @@ def batched_vs_looped(count=512, loop_every=8, seed=7, graph=True, batch_only=Fa
     return {"clouds": count, "points": total_pts, "ridge_points": out_counts,
             "batched_clouds_per_s": batched_rate, "looped_clouds_per_s": loop_rate,
             "batched_over_looped": batched_rate / loop_rate,
-            "looped_sample": f"every {loop_every}th cloud ({len(sample)} clouds, same size distribution)",
-            "batched_s": t_batch, "batched_by_dimension_s": t_split, "looped_s": t_loop}
+            "looped_sample": (f"all {len(sample)} clouds" if loop_every == 1 else
+                              f"every {loop_every}th cloud ({len(sample)} clouds, same size distribution)"),
+            "batched_s": t_batch, "batched_by_dimension_s": t_split, "looped_s": t_loop, "scan_shape": shape}
 
 
 def batched_device_generated(count=512, seed=7):
@@ -194,8 +279,9 @@ def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
     """Differentiated evolve on the 200k-point helix (sigma 0.05, r 0.2): forward pass of
     `iters` iterations keeping checkpoints every `every` iterations, then the backward pass
     (recomputation of the earlier runs + the VJP kernels) for the helix loss gradient.
-    Times are the library's own (device buffers; the host staging of the gradient arrays is
-    excluded), against the forward and against plain evolve iterations (no checkpoints)."""
+    Times are the library's own wall clock (device buffers; the host staging of the gradient
+    arrays is excluded), against the forward and against plain evolve iterations (no
+    checkpoints, evolve_fixed, timed by the same library clock)."""
     from ridgedetection_amd import synth
     from ridgedetection_amd._lib import Context
     P = synth.helix3d(n, seed=2024, dtype=np.float32)
@@ -207,6 +293,8 @@ def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
     _, G = synth.helix_loss(ctx.points())
     G = G.astype(np.float32)
     ctx.evolve_bwd(G)
+    ctx.choose(r)
+    ctx.evolve_fixed(r, iters)                 # warm-up of the plain form
     fwd, bwd, plain = [], [], []
     for _ in range(reps):
         ctx.choose(r)
@@ -217,14 +305,143 @@ def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
         fwd.append(t2["forward_ms"] - t["forward_ms"])
         bwd.append(t2["backward_ms"] - t["backward_ms"])
         ctx.choose(r)
-        t0 = time.perf_counter()
+        t = ctx.timers()
         ctx.evolve_fixed(r, iters)
-        plain.append((time.perf_counter() - t0) * 1e3)
+        plain.append(ctx.timers()["evolve_ms"] - t["evolve_ms"])   # the library's clock, as forward_ms
     ctx.close()
     f, b, p = (float(np.median(x)) for x in (fwd, bwd, plain))
     return {"points": n, "chosen": K, "iterations": iters, "checkpoint_every": every,
             "forward_ms": f, "backward_ms": b, "plain_evolve_ms": p,
             "fwd_plus_bwd_over_fwd": (f + b) / f, "fwd_plus_bwd_over_plain_evolve": (f + b) / p}
+
+
+def fp64_helix(n=1_000_000, steps=10, warmup=3, r=0.2):
+    """The headline workload in FP64 (same seeded helix, cast): scan kernel and whole
+    iteration times, against the FP64 vector peak."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context, OPT_PROFILE
+    P = synth.helix3d(n, seed=2024, sigma=0.05, turns=5.0, pitch=0.15, dtype=np.float64)
+    with Context(3, np.float64) as c:
+        c.set_points(P)
+        S0, _ = c.choose(r)
+        K = len(S0)
+        c.set_option(OPT_PROFILE, 1)
+        c.evolve_fixed(r, warmup)
+        tb = c.timers()
+        t0 = time.perf_counter()
+        c.evolve_fixed(r, steps)
+        el = time.perf_counter() - t0
+        ta = c.timers()
+    launches = max(ta["scan_launches"] - tb["scan_launches"], 1)
+    scan_ms = (ta["scan_kernel_ms"] - tb["scan_kernel_ms"]) / launches
+    iter_ms = (ta["iteration_ms"] - tb["iteration_ms"]) / launches
+    flops = n * K * 9 + n + 3 * n
+    return {"points": n, "chosen": K, "pair_evals_per_s": n * K * steps / el, "ms_per_step": el * 1e3 / steps,
+            "scan_kernel_ms": scan_ms, "evolve_iteration_device_ms": iter_ms,
+            "scan_TFLOPs": flops / (scan_ms * 1e-3) / 1e12,
+            "scan_frac_of_fp64_peak": flops / (scan_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "scan_shape": {"samples_per_lane": ta["scan_ppt"], "threads_per_block": ta["scan_threads"]}}
+
+
+def fp32_vs_fp64():
+    """The error of FP32 results against FP64 on the same coordinates.  End to end (the ridge
+    is a discontinuous function of the input, so the two can differ by a flipped near tie):
+    ridge point counts and the Hausdorff distance between the FP32 and FP64 ridges of the 20k
+    spiral.  At frozen memberships (rounding only): 20 FP32 evolve iterations of the 200k
+    helix on the GPU against the same iterations replayed in FP64 (numpy) with the GPU's labels
+    and moved coordinates, max |S32 - S64| / extent."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context
+    out = {}
+    P = synth.spiral2d(20000, seed=1)
+    res = {}
+    for dt in (np.float32, np.float64):
+        with Context(2, dt) as c:
+            c.set_points(P.astype(dt))
+            res[dt] = c.ridge_detection(0.45, 0.9, max_evolve_iters=100)
+    with Context(2, np.float64) as c:
+        h = c.hausdorff(res[np.float32].astype(np.float64), res[np.float64])
+    ext = float(np.max(P.max(axis=0) - P.min(axis=0)))
+    out["spiral20k"] = {"fp32_points": len(res[np.float32]), "fp64_points": len(res[np.float64]),
+                        "hausdorff": h, "hausdorff_over_extent": h / ext}
+    P = synth.helix3d(200000, seed=11)
+    r, T = 0.2, 20
+    with Context(3, np.float32) as c:
+        c.set_points(P)
+        S0, _ = c.choose(r)
+        S = S0
+        labels, moved = [], []
+        for _ in range(T):
+            S, _, _, _, lab, mv = c.evolve_iteration(r)
+            labels.append(lab)
+            moved.append(mv)
+    P64 = P.astype(np.float64)
+    S64 = S0.astype(np.float64)
+    K, d = S64.shape
+    for lab, mv in zip(labels, moved):
+        inside = lab >= 0
+        cnt = np.bincount(lab[inside], minlength=K).astype(np.float64)
+        for j in range(d):
+            sums = np.bincount(lab[inside], weights=P64[inside, j], minlength=K)
+            upd = ((mv >> j) & 1).astype(bool) & (cnt > 0)
+            S64[upd, j] = sums[upd] / cnt[upd]
+    ext = float(np.max(P.max(axis=0) - P.min(axis=0)))
+    out["helix200k_frozen_memberships"] = {"iterations": T, "chosen": K,
+                                           "max_abs_err_over_extent": float(np.max(np.abs(S - S64))) / ext,
+                                           "bound": 1e-5}
+    return out
+
+
+def tolerance_mode():
+    """Tolerance mode (parallel sums in a wider type, RDHIP_OPT_SUM_MODE = 1) against the exact
+    mode on the workloads whose wall time sits on the serial-sum floor: the 512-cloud batch and
+    the 6-D Bezier grouped tiling.  Times, ridge point counts and the Hausdorff distance of each
+    tolerance-mode ridge to the exact one (relative to the evolve radius)."""
+    from ridgedetection_amd import synth
+    from ridgedetection_amd._lib import Context, OPT_SUM_MODE
+    out = {}
+    clouds = synth.batched_clouds(count=512, seed=7)
+    dmax = max(p.shape[1] for p, _, _ in clouds)
+    res, secs = {}, {}
+    for mode in (0, 1):
+        with Context(dmax, np.float32) as c:
+            c.set_option(OPT_SUM_MODE, mode)
+            small = sorted(range(len(clouds)), key=lambda i: len(clouds[i][0]))[:4]
+            c.batch([clouds[i][0] for i in small], [clouds[i][1] for i in small], [clouds[i][2] for i in small])
+            t0 = time.perf_counter()
+            res[mode] = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
+            secs[mode] = time.perf_counter() - t0
+        log(f"bench: batch sum mode {mode}: {secs[mode]:.2f} s")
+    rel = []
+    with Context(3, np.float64) as c:
+        for (p, r1, _), ex, to in zip(clouds, res[0], res[1]):
+            if len(ex) and len(to):
+                a = np.zeros((len(to), 3)); a[:, :p.shape[1]] = to
+                b = np.zeros((len(ex), 3)); b[:, :p.shape[1]] = ex
+                rel.append(c.hausdorff(a, b) / r1)
+    out["batched_512_clouds"] = {
+        "exact_s": secs[0], "tolerance_s": secs[1], "speedup": secs[0] / secs[1],
+        "exact_clouds_per_s": 512 / secs[0], "tolerance_clouds_per_s": 512 / secs[1],
+        "ridge_points_exact": int(sum(len(x) for x in res[0])), "ridge_points_tolerance": int(sum(len(x) for x in res[1])),
+        "clouds_with_equal_counts": int(sum(len(a) == len(b) for a, b in zip(res[0], res[1]))),
+        "hausdorff_over_r1_median": float(np.median(rel)), "hausdorff_over_r1_max": float(np.max(rel))}
+    P = synth.bezier_union(2_000_000, dim=6, seed=6, dtype=np.float32)
+    tres, tt = {}, {}
+    for mode in (0, 1):
+        with Context(6, np.float32) as c:
+            c.set_option(OPT_SUM_MODE, mode)
+            c.set_points(P)
+            t0 = time.perf_counter()
+            tres[mode] = c.tiled(1, 0.08, 0.16, 10000, [4] * 6)
+            tt[mode] = (time.perf_counter() - t0, c.timers()["evolve_ms"])
+        log(f"bench: bezier6 grouped sum mode {mode}: {tt[mode][0]:.2f} s")
+    with Context(6, np.float64) as c:
+        h = c.hausdorff(tres[1].astype(np.float64), tres[0].astype(np.float64))
+    out["bezier6/grouped tree, mixed RD"] = {
+        "exact_wall_s": tt[0][0], "tolerance_wall_s": tt[1][0], "exact_evolve_ms": tt[0][1],
+        "tolerance_evolve_ms": tt[1][1], "ridge_points_exact": len(tres[0]), "ridge_points_tolerance": len(tres[1]),
+        "hausdorff_over_r1": h / 0.08}
+    return out
 
 
 TILED_NAMES = {0: "local tree, mixed RD", 1: "grouped tree, mixed RD", 2: "local tree, local RD",
@@ -259,12 +476,19 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
             wall = time.perf_counter() - t0
             t = ctx.timers()
             ctx.close()
-            gbs = t["evolve_bytes"] / (t["evolve_ms"] * 1e-3) / 1e9 if t["evolve_ms"] > 0 else 0.0
+            sec = t["evolve_ms"] * 1e-3
+            gbs = t["evolve_bytes"] / sec / 1e9 if sec > 0 else 0.0
+            pps = t["evolve_pairs"] / sec if sec > 0 else 0.0
+            tflops = pps * 3 * P.shape[1] / 1e12        # the reference's 3d flops per pair
             out[f"{w}/{TILED_NAMES[alg]}"] = {
                 "points": len(P), "dim": P.shape[1], "ridge_points": len(res), "wall_s": wall,
                 "build_ms": t["build_ms"], "choose_ms": t["choose_ms"], "evolve_ms": t["evolve_ms"],
                 "decimate_ms": t["decimate_ms"], "evolve_iterations": t["evolve_iterations"],
-                "rounds": t["rounds"], "evolve_GBps": gbs, "evolve_frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+                "rounds": t["rounds"], "evolve_GBps": gbs, "evolve_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,
+                "evolve_pair_evals_per_s": pps, "evolve_TFLOPs": tflops,
+                "evolve_frac_of_fp32_valu_peak": tflops / FP32_PEAK_TFLOPS,
+                "scan_shape": {"samples_per_lane": t["scan_ppt"], "threads_per_block": t["scan_threads"],
+                               "persistent": bool(t["scan_persistent"])}}
     return out
 
 
@@ -331,6 +555,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--ppt", type=int, default=0, help="samples per lane (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-configs", action="store_true",
+                    help="skip the host's whole ridge detections per workload (the evolve-pass baseline stays)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-cloud batched measurement")
     ap.add_argument("--no-vjp", action="store_true", help="skip the differentiated-evolve measurement")
     ap.add_argument("--no-tiled", action="store_true", help="skip the tiled measurements")
@@ -431,6 +657,10 @@ def main():
         "secondary": secondary,
     }
     ctx.close()
+    if dt == np.float32:
+        log("bench: FP64 helix and FP32-vs-FP64 errors")
+        result["secondary"]["fp64_helix_1M"] = fp64_helix(n=args.n)
+        result["secondary"]["fp32_vs_fp64"] = fp32_vs_fp64()
     if not args.no_vjp and dt == np.float32:
         log("bench: differentiated evolve (200k helix, 20 iterations)")
         result["secondary"]["vjp_helix_200k"] = vjp_helix()
@@ -439,9 +669,14 @@ def main():
     if not args.no_batch:
         result["secondary"]["batched_512_device_generated"] = batched_device_generated()
         result["secondary"]["batched_512_clouds"] = batched_vs_looped()
+        if dt == np.float32 and not args.no_tiled:
+            result["secondary"]["tolerance_mode"] = tolerance_mode()
     if not args.no_cpu:
         log("bench: CPU baseline")
         result["cpu_baseline"] = cpu_baseline(P, S0, r)
+        if dt == np.float32 and not args.no_cpu_configs:
+            log("bench: CPU whole ridge detection per workload")
+            result["secondary"]["cpu_per_config"] = cpu_per_config()
     print(json.dumps(result), flush=True)
 
 

@@ -95,6 +95,10 @@ typedef struct rdhip_timers {
     int64_t scan_ppt;        /* evolve scan shape in use: samples per lane ... */
     int64_t scan_threads;    /* ... and threads per block (after autotuning) */
     double tune_ms;          /* time spent timing candidate scan shapes */
+    double evolve_pairs;     /* (sample, chosen point) pairs the evolve iterations evaluated:
+                                per segment and iteration, its samples x its chosen points */
+    int64_t scan_persistent; /* 1: the scan's blocks pull work items from a device queue */
+    int64_t vjp_checkpoint_every;  /* iterations per recorded run of the last rdhip_evolve_fwd */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */
@@ -103,8 +107,9 @@ enum rdhip_option {
                                       turns the autotuner off) */
     RDHIP_OPT_ITERS_PER_SYNC = 1,  /* evolve iterations queued between convergence checks (8) */
     RDHIP_OPT_PROFILE = 2,         /* 1: time the scan kernel and iterations with HIP events */
-    RDHIP_OPT_PERSISTENT = 3,      /* 1 (default): persistent scan blocks fed by a device work
-                                      queue; 0: one block per work item */
+    RDHIP_OPT_PERSISTENT = 3,      /* batches: 1 = persistent scan blocks fed by a device work
+                                      queue, 0 = one block per work item (setting it turns the
+                                      autotuner off; by default the autotuner picks) */
     RDHIP_OPT_GRAPH = 4,           /* 1 (default): replay captured batches of evolve iterations
                                       (hipGraph) until the device convergence flag is set */
     RDHIP_OPT_LOCAL = 5,           /* 1 (default): small segments (at most 1024 chosen points and
@@ -116,11 +121,14 @@ enum rdhip_option {
     RDHIP_OPT_LOCAL_MAX_WORK = 7,     /* limit on samples x chosen points (default 2^22) */
     RDHIP_OPT_SCAN_THREADS = 8,    /* threads per evolve-scan block: 128, 256 or 512 (setting it
                                       turns the autotuner off) */
-    RDHIP_OPT_AUTOTUNE = 9,        /* 1 (default): a single cloud's evolve, on first use for a
-                                      (value type, dimension, log2 samples, log2 chosen points)
-                                      class, times every (samples per lane x threads per block)
-                                      scan shape on the data and keeps the fastest (cached per
-                                      process); batches keep 4 x 256 */
+    RDHIP_OPT_AUTOTUNE = 9,        /* 1 (default): on first use for a (value type, dimension,
+                                      log2 samples, log2 chosen points) class, a single cloud's
+                                      evolve times every (samples per lane x threads per block)
+                                      scan shape on the data and keeps the fastest; a batch of
+                                      segments (clouds, tiles) does the same per (value type,
+                                      dimension, log2 samples, log2 segments) class over nine
+                                      (samples per lane x threads x persistent queue or not)
+                                      shapes.  Cached per process */
     RDHIP_OPT_SEGMENTED = 10,      /* 1: a batch whose active segments all have at most 65536
                                       samples and 1024 chosen points iterates as a batch-wide scan
                                       plus one workgroup per segment for the member sort, ordered
@@ -133,9 +141,19 @@ enum rdhip_option {
                                       with RDHIP_ERR_RUNTIME instead of running on (default 100000;
                                       0 = no limit, the reference's behaviour, which never returns
                                       on an input that does not converge) */
-    RDHIP_OPT_INJECT_ALLOC_FAILURE = 13  /* test hook: the value-th device allocation the calling
+    RDHIP_OPT_INJECT_ALLOC_FAILURE = 13, /* test hook: the value-th device allocation the calling
                                       thread makes from now on (in any context) fails with
                                       hipErrorOutOfMemory; 0 = off */
+    RDHIP_OPT_VJP_MEMORY_MB = 14,  /* memory budget (MiB, default 1024) of rdhip_evolve_fwd with
+                                      checkpoint_every < 0 */
+    RDHIP_OPT_SUM_MODE = 15        /* 0 (default): every centroid sum in the reference's serial
+                                      sample order, results bit-identical to the reference CPU
+                                      path; 1 (tolerance mode): batch-wide iterations sum each chosen
+                                      point's members in parallel in a wider type (double for float
+                                      data, double-double for double data) in a fixed order and take
+                                      the mean of that sum -- no serial floor on long chains,
+                                      deterministic, but means differ from the reference's by its
+                                      own rounding of the serial sum */
 };
 
 int rdhip_version(void);
@@ -309,7 +327,10 @@ int rdhip_order_chains(rdhip_ctx* ctx, double r2, int32_t* order, int64_t* chain
  * at the start of every run of `checkpoint_every` iterations, and the memberships of the
  * last run.  checkpoint_every = 0 keeps the memberships of every iteration (iters x n
  * int32 of device memory); otherwise earlier runs are recomputed from their checkpoints
- * during the backward pass (about one more forward pass of work).
+ * during the backward pass (about one more forward pass of work).  checkpoint_every < 0
+ * picks the interval: every iteration's memberships when iters x (4n + 4K + dim K) bytes plus
+ * the checkpoints fit RDHIP_OPT_VJP_MEMORY_MB, else the longest run that does
+ * (rdhip_timers.vjp_checkpoint_every reports it).
  *
  * rdhip_evolve_bwd is the vector-Jacobian product of that forward pass with the
  * memberships (labels, counts and moved coordinates of every iteration) held fixed:

@@ -505,6 +505,9 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->scan_ppt = e.timers.scanPpt;
         out->scan_threads = e.timers.scanThreads;
         out->tune_ms = e.timers.tuneMs;
+        out->evolve_pairs = e.timers.evolvePairs;
+        out->scan_persistent = e.timers.scanPersistent;
+        out->vjp_checkpoint_every = e.timers.vjpCheckpointEvery;
         return RDHIP_OK;
     });
 }
@@ -620,6 +623,7 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
                 return RDHIP_OK;
             case RDHIP_OPT_PERSISTENT:
                 e.options.persistent = value != 0;
+                e.options.autotune = false;
                 return RDHIP_OK;
             case RDHIP_OPT_GRAPH:
                 e.options.graph = value != 0;
@@ -634,6 +638,14 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
             case RDHIP_OPT_LOCAL_MAX_WORK:
                 if (value < 0) return fail(RDHIP_ERR_INVALID, "negative local work limit");
                 e.options.localMaxWork = value;
+                return RDHIP_OK;
+            case RDHIP_OPT_SUM_MODE:
+                if (value != 0 && value != 1) return fail(RDHIP_ERR_INVALID, "sum mode must be 0 (exact) or 1 (tolerance)");
+                e.options.tolerance = value == 1;
+                return RDHIP_OK;
+            case RDHIP_OPT_VJP_MEMORY_MB:
+                if (value < 1) return fail(RDHIP_ERR_INVALID, "the VJP memory budget must be at least 1 MiB");
+                e.options.vjpBudgetBytes = value << 20;
                 return RDHIP_OK;
             case RDHIP_OPT_EVOLVE_GUARD:
                 if (value < 0) return fail(RDHIP_ERR_INVALID, "negative iteration guard");
@@ -794,12 +806,11 @@ int rdhip_order_chains(rdhip_ctx* ctx, double r2, int32_t* order, int64_t* chain
 int rdhip_evolve_fwd(rdhip_ctx* ctx, double r, int64_t iters, int64_t checkpoint_every) {
     if (!(r >= 0)) return fail(RDHIP_ERR_INVALID, "radius must be non-negative");
     if (iters < 1) return fail(RDHIP_ERR_INVALID, "iteration count must be positive");
-    if (checkpoint_every < 0) return fail(RDHIP_ERR_INVALID, "negative checkpoint interval");
     return withEngine(ctx, [&](auto& e, auto zero) -> int {
         using T = decltype(zero);
         if (!ctx->hasChosen) return fail(RDHIP_ERR_STATE, "no chosen points (call rdhip_choose first)");
         e.setRadii({(T)r}, {(T)r});
-        e.evolveForward(iters, (int)std::min<int64_t>(checkpoint_every, iters));
+        e.evolveForward(iters, (int)std::max<int64_t>(std::min<int64_t>(checkpoint_every, iters), -1));
         return RDHIP_OK;
     });
 }

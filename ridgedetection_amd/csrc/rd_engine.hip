@@ -627,6 +627,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
                     log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
                     log ? log->moved : (record ? moved_.get() : nullptr),
                     log ? log->movedStride : totalSlots_, longList_.get(), longShift, nullptr};
+    ha.tolerance = options.tolerance ? 1 : 0;
     if (rowsValid) {
         // rows over the live slots of the evolve call only
         ha.totalSlots = numRowSlots_;
@@ -664,6 +665,12 @@ void Engine<T>::readEvents(int used) {
 namespace {
 std::mutex g_tuneMutex;
 std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> g_tuned;   // -> (ppt, threads)
+struct BatchShape {
+    int ppt;
+    int threads;
+    bool persistent;
+};
+std::map<std::tuple<int, int, int, int, int>, BatchShape> g_tunedBatch;
 inline int ilog2(int64_t v) {
     int l = 0;
     while (v > 1) {
@@ -674,14 +681,104 @@ inline int ilog2(int64_t v) {
 }
 }  // namespace
 
+// Batches of segments: the compact scan over every segment's work items, timed for each
+// (samples per lane x threads per block x persistent queue or one block per item) shape,
+// three rounds, best round kept; cached per (value type, dimension, log2 samples, log2 chosen
+// points, log2 segments).  Items then follow the picked shape (its size is also the
+// granularity of the active-item list).
+template <typename T>
+void Engine<T>::autotuneBatchScan() {
+    auto t0 = std::chrono::steady_clock::now();
+    const std::vector<int> K = countsHost();
+    int64_t kTotal = 0;
+    for (int k : K) kTotal += k;
+    const int ns = (int)segs_.size();
+    const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)), 0, 1 + ilog2(ns));
+    BatchShape best{options.ppt, options.scanThreads, options.persistent};
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> g(g_tuneMutex);
+        auto it = g_tunedBatch.find(key);
+        if (it != g_tunedBatch.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        static const BatchShape cands[] = {{4, 256, true},  {2, 256, true},  {1, 256, true},
+                                           {4, 256, false}, {2, 256, false}, {1, 256, false},
+                                           {2, 512, false}, {1, 512, false}, {4, 128, false}};
+        constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
+        const EvolveSource es = evolveSource();
+        const std::vector<int> none(ns, 0);
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        float score[kCands];
+        for (int c = 0; c < kCands; ++c) score[c] = 1e30f;
+        // two rounds of one untimed and one timed launch per shape (one round when a launch
+        // takes more than 4 ms: the tuning then costs about 20 scans of the batch)
+        int rounds = 2;
+        for (int round = 0; round < rounds; ++round) {
+            for (int c = 0; c < kCands; ++c) {
+                options.ppt = cands[c].ppt;
+                options.scanThreads = cands[c].threads;
+                options.persistent = cands[c].persistent;
+                buildWorkItems();
+                orderItemsByCost();
+                rebuildActive(none);
+                ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+                               dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
+                               cKeys_.get(), cPos_.get(), flags_.get() + 4, nullptr};
+                const int pb = persistentBlocks();
+                check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, pb, st_), "tune");
+                check(hipEventRecord(e0, st_), "event");
+                check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, pb, st_), "tune");
+                check(hipEventRecord(e1, st_), "event");
+                check(hipEventSynchronize(e1), "event");
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, e0, e1), "event");
+                score[c] = std::min(score[c], ms);
+                if (round == 0 && c == 0 && ms > 4.0f) rounds = 1;
+            }
+        }
+        float bestMs = 1e30f;
+        for (int c = 0; c < kCands; ++c)
+            if (score[c] < bestMs) {
+                bestMs = score[c];
+                best = cands[c];
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> g(g_tuneMutex);
+        g_tunedBatch[key] = best;
+    }
+    if (!cached || options.ppt != best.ppt || options.scanThreads != best.threads ||
+        options.persistent != best.persistent) {
+        options.ppt = best.ppt;
+        options.scanThreads = best.threads;
+        options.persistent = best.persistent;
+        buildWorkItems();
+        orderItemsByCost();
+        check(hipStreamSynchronize(st_), "autotune");
+    }
+    timers.scanPpt = options.ppt;
+    timers.scanThreads = options.scanThreads;
+    timers.scanPersistent = options.persistent ? 1 : 0;
+    timers.tuneMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 template <typename T>
 void Engine<T>::autotuneScan() {
     const EvolveSource es = evolveSource();
-    // Batches of segments keep the default shape: their iterations are dominated by the
-    // per-iteration passes, and the item size also sets the granularity of the active list.
+    if (options.autotune && segs_.size() > 1) {
+        autotuneBatchScan();
+        return;
+    }
     if (!options.autotune || segs_.size() != 1) {
         timers.scanPpt = options.ppt;
         timers.scanThreads = options.scanThreads;
+        timers.scanPersistent = options.persistent ? 1 : 0;
         return;
     }
     auto t0 = std::chrono::steady_clock::now();
@@ -757,11 +854,14 @@ void Engine<T>::autotuneScan() {
 template <typename T>
 void Engine<T>::addEvolveBytes(const std::vector<int>& itersPerSeg) {
     std::vector<int> K = countsHost();
-    double b = 0;
-    for (size_t s = 0; s < segs_.size(); ++s)
+    double b = 0, pairs = 0;
+    for (size_t s = 0; s < segs_.size(); ++s) {
         b += (double)itersPerSeg[s] *
              ((double)segs_[s].nP * dim_ * sizeof(T) + (double)K[s] * (3.0 * dim_ * sizeof(T) + 4.0));
+        pairs += (double)itersPerSeg[s] * (double)segs_[s].nP * (double)K[s];
+    }
     timers.evolveBytes += b;
+    timers.evolvePairs += pairs;
 }
 
 template <typename T>
@@ -1068,7 +1168,23 @@ void Engine<T>::evolveForward(long long iters, int every) {
     ++version_;
     auto t0 = std::chrono::steady_clock::now();
     tape_.valid = false;
+    if (every < 0) {
+        // automatic: the longest run whose memberships (labels, counts, moved flags) plus the
+        // chosen-set checkpoints fit the memory budget; every iteration's when they all fit
+        const double perIter = 4.0 * (double)totalP_ + (4.0 + dim_) * std::max(countsHost()[0], 1);
+        const double ckpt = (double)sizeof(T) * (double)sStride_ * dim_;
+        const double budget = (double)options.vjpBudgetBytes;
+        every = 1;
+        for (long long e = iters; e >= 1; e = e > 64 ? e - (e + 63) / 64 : e - 1) {
+            const double need = (double)e * perIter + (double)((iters + e - 1) / e) * ckpt;
+            if (need <= budget) {
+                every = (int)e;
+                break;
+            }
+        }
+    }
     if (every <= 0 || every > iters) every = (int)iters;
+    timers.vjpCheckpointEvery = every;
     tape_.iters = iters;
     tape_.every = every;
     tape_.runs = (int)((iters + every - 1) / every);

@@ -70,7 +70,10 @@ struct Timers {
     // read once (nP * dim * sizeof(T)) and its chosen points read, summed and updated
     // (K * (3 * dim * sizeof(T) + 4))
     double evolveBytes = 0;
+    double evolvePairs = 0;             // samples x chosen points, summed over segments and iterations
     int scanPpt = 0, scanThreads = 0;   // scan shape in use (after autotuning)
+    int scanPersistent = 0;             // ... and whether its blocks pull items from a queue
+    long long vjpCheckpointEvery = 0;   // iterations per recorded run of the last VJP forward
     double tuneMs = 0;                  // time spent timing candidate shapes
 };
 
@@ -91,6 +94,9 @@ struct RunOptions {
     bool segmented = false;         // batches whose active segments all have <= segmentedMaxP
     int segmentedMaxP = 65536;      // samples: batch-wide scan + per-segment sort and sums
     bool positionCopy = true;       // indexed segments: evolve reads a position-ordered copy
+    int64_t vjpBudgetBytes = int64_t(1) << 30;   // automatic checkpoint interval of the VJP forward
+    bool tolerance = false;         // batch-wide sums in parallel in a wider type (not the
+                                    // reference's serial order); segment-local ones stay exact
     long long evolveGuard = 100000; // an evolve still moving after this many iterations fails
                                     // (0 = no limit, as in the reference)
 };
@@ -282,6 +288,7 @@ private:
         int64_t movedStride;
     };
     // finalize: also the per-segment convergence bookkeeping (iteration cap capIter)
+    void autotuneBatchScan();
     void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr, bool finalize = false,
                           int capIter = 0);
     // state of the last evolveForward

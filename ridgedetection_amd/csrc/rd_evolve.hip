@@ -206,6 +206,95 @@ __device__ __forceinline__ T group_ordered_sum(const T* __restrict__ src, int cn
     return sum;
 }
 
+// ---------------------------------------------------------------------------------
+// 4b''. tolerance mode: parallel sums in a wider type.  Lane l of a group of kLanes lanes
+// accumulates members l, l + kLanes, ... into four interleaved accumulators (so the adds do
+// not wait on each other), combined in a fixed order and then across the group by a fixed
+// xor butterfly: the result does not depend on scheduling.  Float data accumulates in
+// double: the error of a sum of m members is at most about m 2^-53 times the sum of their
+// magnitudes, below one float unit of that magnitude for m < 2^29.  Double data accumulates in
+// double-double (Knuth's TwoSum), accurate to a few units of the last place of a double
+// whatever the chain length.
+// ---------------------------------------------------------------------------------
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD dd_add(DD a, double b) {
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double err = (a.hi - (s - bb)) + (b - bb);
+    return DD{s, a.lo + err};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    DD r = dd_add(a, b.hi);
+    r.lo += b.lo;
+    return r;
+}
+template <typename T>
+struct WideSum;
+template <>
+struct WideSum<float> {
+    using Acc = double;
+    static __device__ __forceinline__ Acc zero() { return 0.0; }
+    static __device__ __forceinline__ Acc add(Acc a, float x) { return a + (double)x; }
+    static __device__ __forceinline__ Acc add(Acc a, Acc b) { return a + b; }
+    static __device__ __forceinline__ Acc xorShfl(Acc a, int m) { return __shfl_xor(a, m); }
+    static __device__ __forceinline__ double value(Acc a) { return a; }
+};
+template <>
+struct WideSum<double> {
+    using Acc = DD;
+    static __device__ __forceinline__ Acc zero() { return DD{0.0, 0.0}; }
+    static __device__ __forceinline__ Acc add(Acc a, double x) { return dd_add(a, x); }
+    static __device__ __forceinline__ Acc add(Acc a, Acc b) { return dd_add(a, b); }
+    static __device__ __forceinline__ Acc xorShfl(Acc a, int m) { return DD{__shfl_xor(a.hi, m), __shfl_xor(a.lo, m)}; }
+    static __device__ __forceinline__ double value(Acc a) { return a.hi + a.lo; }
+};
+
+// Sum of src[0 .. cnt) by a group of kLanes lanes (all lanes of the group return it).
+template <typename T, int kLanes>
+__device__ __forceinline__ typename WideSum<T>::Acc group_parallel_sum(const T* __restrict__ src, int cnt) {
+    using W = WideSum<T>;
+    const int lane = threadIdx.x & (kLanes - 1);
+    typename W::Acc acc[4] = {W::zero(), W::zero(), W::zero(), W::zero()};
+    int i = lane;
+    for (; i + 3 * kLanes < cnt; i += 4 * kLanes) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = W::add(acc[u], src[i + u * kLanes]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (i + u * kLanes < cnt) acc[u] = W::add(acc[u], src[i + u * kLanes]);
+    typename W::Acc s = W::add(W::add(acc[0], acc[1]), W::add(acc[2], acc[3]));
+#pragma unroll
+    for (int m = kLanes / 2; m >= 1; m /= 2) s = W::add(s, W::xorShfl(s, m));
+    return s;
+}
+
+// The shift rule of shift_coordinate with the mean taken from a wide sum.
+template <typename T>
+__device__ __forceinline__ void shift_coordinate_wide(const ShiftArgs<T>& a, int d, int64_t slot, int seg, int cnt,
+                                                      double wide) {
+    if (a.outCounts && d == 0) a.outCounts[slot] = cnt;
+    if (a.outSums) a.outSums[d * a.sStride + slot] = (T)wide;
+    bool moved = false;
+    if (cnt != 0) {
+        const T fc = (T)cnt;
+        constexpr T eps = FpTraits<T>::eps;
+        const T mc = (T)(wide / (double)cnt);
+        T thr = T(2) * fabs(mc);
+        thr = thr * fc;
+        thr = thr * eps;
+        T& s = a.S[d * a.sStride + slot];
+        if (fabs(mc - s) > thr) {
+            s = mc;
+            moved = true;
+        }
+    }
+    if (a.outMoved) a.outMoved[d * a.movedStride + slot] = moved ? 1 : 0;
+    if (moved) a.segMoved[seg] = 1;
+}
+
 // 16-lane row path: one row per (coordinate, chosen slot) of chains not on the long list
 template <typename T, int DIM>
 __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t blk, T* waveLds) {
@@ -235,8 +324,14 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
     // the wave steps through the longest of its four chains; every lane takes part in the
     // staging.  Loads are unconditional (index clamped into the chain, or element 0 of the
     // buffer for an empty one).
-    const int steps = wave_row_max(cnt);
     const T* __restrict__ src = a.sortedP + (int64_t)min(d, DIM - 1) * a.mStride + off;
+    if (a.tolerance) {
+        const double wide = WideSum<T>::value(group_parallel_sum<T, kRow>(src, cnt));
+        if (seg < 0 || t != 0) return;
+        shift_coordinate_wide(a, d, slot, seg, cnt, wide);
+        return;
+    }
+    const int steps = wave_row_max(cnt);
     const int rowInWave = (threadIdx.x & (kWave - 1)) / kRow;
     const T sum = group_ordered_sum<T, kRow>(src, cnt, steps, waveLds + rowInWave * (kRowTile + WideOf<T>::n));
     if (seg < 0 || t != 0) return;
@@ -256,6 +351,11 @@ __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wa
     if (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg]) return;
     const int off = a.chainOff[key];
     const int cnt = a.chainEnd[key] - off;
+    if (a.tolerance) {
+        const double wide = WideSum<T>::value(group_parallel_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt));
+        if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate_wide(a, d, slot, seg, cnt, wide);
+        return;
+    }
     const T sum = group_ordered_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt, cnt, tile);
     if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
 }

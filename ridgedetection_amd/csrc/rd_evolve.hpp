@@ -82,6 +82,10 @@ struct ShiftArgs {
     // chains are indexed by the dense key (row r's chain is r, a listed chain's slot is
     // rowSlots[key]) instead of the slot
     int denseChains;
+    // 1: tolerance mode -- every chain is summed in parallel in a wider type (double for
+    // float data, double-double for double data) in a fixed order, instead of serially in the
+    // reference's sample order; the mean is that sum over the count, rounded once
+    int tolerance;
 };
 
 // L of the wave-summed chains for up to maxM sorted members: the smallest power of two

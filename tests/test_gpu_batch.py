@@ -133,3 +133,19 @@ def test_segment_local_and_batchwide_iterations_agree(oracle, local, segmented):
             c.set_points(P)
             S = c.tiled(alg, 0.2, 0.4, 3000, [4, 4, 4])
         np.testing.assert_array_equal(S, oracle.tiled(alg, P, 0.2, 0.4, 3000, [4, 4, 4]))
+
+
+def test_large_clouds_batch_rule(oracle):
+    """Clouds of 100k-300k points drawn by the batched workload's own rule (spiral / helix /
+    segment chains, noise sigma ~ U[0.01, 0.1] x extent, r1 = 4 sigma, r2 = 2 r1), two 2-D and
+    two 3-D, in one mixed-dimension batch with the autotuned batch scan shape: each cloud's
+    ridge equals the oracle's single-cloud result."""
+    from ridgedetection_amd._lib import Context
+    clouds = synth.batched_clouds(count=4, seed=11, min_pts=100000, max_pts=300000)
+    print("case batch of 4 clouds, " + ", ".join(f"{len(p)} x {p.shape[1]}" for p, _, _ in clouds), flush=True)
+    with Context(3, np.float32) as c:
+        res = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
+        t = c.timers()
+    print(f"  scan shape {t['scan_ppt']} x {t['scan_threads']} persistent={t['scan_persistent']}", flush=True)
+    for (P, r1, r2), S in zip(clouds, res):
+        np.testing.assert_array_equal(S, oracle.ridge_detection(P, r1, r2)[0])

@@ -78,6 +78,30 @@ def test_checkpointing_is_exact():
         np.testing.assert_array_equal(gs0, outs[0][1])
 
 
+def test_automatic_checkpoint_interval():
+    """checkpoint_every < 0: every iteration's memberships when they fit the memory budget,
+    else the longest run that fits; the gradient does not change."""
+    from ridgedetection_amd._lib import OPT_VJP_MEMORY_MB
+    n, T, r = 200000, 20, 0.2
+    P = synth.helix3d(n, seed=8, dtype=np.float32)
+    outs, intervals = [], []
+    # 20 x 200000 x 4 bytes of labels fit 1 GiB; with 16 MiB a run of 14 iterations and two
+    # 2.4 MB checkpoints do, 15 do not
+    for budget_mb in (1024, 16):
+        _say(f"case automatic checkpoints, budget {budget_mb} MiB")
+        with _ctx(P) as c:
+            c.set_option(OPT_VJP_MEMORY_MB, budget_mb)
+            c.choose(r)
+            c.evolve_fwd(r, T, -1)
+            intervals.append(c.timers()["vjp_checkpoint_every"])
+            _, G = synth.helix_loss(c.points())
+            outs.append(c.evolve_bwd(G.astype(np.float32)))
+    assert intervals[0] == T
+    assert 1 < intervals[1] < T and intervals[1] * n * 4 <= 16 << 20
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
 def test_helix_c5_fd_and_fp64(oracle):
     """200 000-point helix, T = 20, checkpoints every 4 iterations: FP64 gradient against
     central differences on 1000 random coordinates; FP32 gradient against the FP64 one."""
