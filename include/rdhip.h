@@ -146,7 +146,7 @@ enum rdhip_option {
                                       hipErrorOutOfMemory; 0 = off */
     RDHIP_OPT_VJP_MEMORY_MB = 14,  /* memory budget (MiB, default 1024) of rdhip_evolve_fwd with
                                       checkpoint_every < 0 */
-    RDHIP_OPT_SUM_MODE = 15        /* 0 (default): every centroid sum in the reference's serial
+    RDHIP_OPT_SUM_MODE = 15,       /* 0 (default): every centroid sum in the reference's serial
                                       sample order, results bit-identical to the reference CPU
                                       path; 1 (tolerance mode): batch-wide iterations sum each chosen
                                       point's members in parallel in a wider type (double for float
@@ -154,6 +154,11 @@ enum rdhip_option {
                                       the mean of that sum -- no serial floor on long chains,
                                       deterministic, but means differ from the reference's by its
                                       own rounding of the serial sum */
+    RDHIP_OPT_CHAIN = 16           /* 1 (default): iterations of one cloud with at most 2048
+                                      chosen points in at most 4 dimensions count the labels in the
+                                      scan, order the members with one scatter and sum every
+                                      (chosen point, coordinate) chain in its own lane; 0: the
+                                      general sort + chain-sum path.  Same results either way */
 };
 
 int rdhip_version(void);

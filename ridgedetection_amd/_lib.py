@@ -39,6 +39,7 @@ TREE_LOCAL, TREE_GROUPED, TREE_EXTENDED = 0, 1, 2
 OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL = 0, 1, 2, 3, 4, 5
 OPT_LOCAL_MAX_SAMPLES, OPT_LOCAL_MAX_WORK, OPT_SCAN_THREADS, OPT_AUTOTUNE, OPT_SEGMENTED = 6, 7, 8, 9, 10
 OPT_POSITION_COPY, OPT_EVOLVE_GUARD, OPT_INJECT_ALLOC_FAILURE, OPT_VJP_MEMORY_MB, OPT_SUM_MODE = 11, 12, 13, 14, 15
+OPT_CHAIN = 16
 GEN_CIRCLE, GEN_SPHERE, GEN_SPIRAL2D, GEN_SPIRAL3D, GEN_SEGMENT, GEN_HELIX, GEN_POLYLINE, GEN_UNIFORM = range(8)
 
 

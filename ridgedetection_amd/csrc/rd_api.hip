@@ -639,6 +639,9 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
                 if (value < 0) return fail(RDHIP_ERR_INVALID, "negative local work limit");
                 e.options.localMaxWork = value;
                 return RDHIP_OK;
+            case RDHIP_OPT_CHAIN:
+                e.options.chain = value != 0;
+                return RDHIP_OK;
             case RDHIP_OPT_SUM_MODE:
                 if (value != 0 && value != 1) return fail(RDHIP_ERR_INVALID, "sum mode must be 0 (exact) or 1 (tolerance)");
                 e.options.tolerance = value == 1;

@@ -87,6 +87,17 @@ __device__ __forceinline__ int64_t point_of(const int* __restrict__ pIndex, int6
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 __device__ __forceinline__ uint64_t lanes_below() { return __lanemask_lt(); }
 
+// XCD-aware block order: the GPU hands consecutive workgroups of a launch to its 8 XCDs in
+// turn, and each XCD has its own L2.  Block i of an n-block launch takes item xcd_item(i, n),
+// a bijection of [0, n) under which the blocks of one XCD take one contiguous run of items,
+// so neighbouring items' partial cache lines meet in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int i, int n) {
+    const int q = n / kXcds, r = n - q * kXcds;
+    const int x = i % kXcds, j = i / kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // Exact squared distance in the reference's operation order (no contraction).
 template <typename T, int DIM>
 __device__ __forceinline__ T exact_sqdist(const T (&p)[DIM], const T (&s)[DIM]) {

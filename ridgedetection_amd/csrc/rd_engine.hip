@@ -421,6 +421,56 @@ void Engine<T>::ensureEvolveScratch() {
     decI64_.reserve(2);
 }
 
+// The histogram stride covers the smallest scan item (128 samples), so every scan shape fits;
+// the member rows leave room for the chains' 16-byte end alignment and the sums' prefetch.
+template <typename T>
+void Engine<T>::ensureChainScratch() {
+    if (!options.chain || segs_.size() != 1 || dim_ > kChainMaxDim) return;
+    const int64_t kb = std::max<int64_t>(keyBound_, 1);
+    if (kb > kChainMaxKeys) return;
+    const int64_t hs = round_up((totalP_ + 127) / 128, 64);
+    const int64_t ms = chain_member_stride(totalP_, kb);
+    chainHist_.reserve((size_t)(kb * hs));
+    chainTotals_.reserve((size_t)kb);
+    if (sortedP_.n < (size_t)(ms * dim_)) sortedP_.reserve((size_t)(ms * dim_));
+    const int64_t entries = long_list_entries(ms, long_chain_shift(dim_, ms));
+    if (longList_.n < (size_t)entries) longList_.reserve((size_t)entries);
+    chainHistStride_ = hs;
+    chainMStride_ = ms;
+    chainKeyCap_ = kb;
+}
+
+template <typename T>
+bool Engine<T>::chainUsable(int itemSize, const IterLog*, bool) const {
+    if (!options.chain || options.tolerance || segs_.size() != 1 || segmentedNow_) return false;
+    const int64_t kb = std::max<int64_t>(keyBound_, 1);
+    if (!chain_iteration_supported(dim_, itemSize, kb) || kb > chainKeyCap_) return false;
+    // the histogram prefix orders blocks by item: items must follow the samples
+    if ((int)items_.size() != numItems_) return false;
+    for (size_t i = 0; i < items_.size(); ++i)
+        if (items_[i].seg != 0 || items_[i].start != (int)i * itemSize) return false;
+    return chainHistStride_ >= numItems_ && chainMStride_ >= chain_member_stride(totalP_, kb) &&
+           sortedP_.n >= (size_t)(chainMStride_ * dim_) && chainHist_.n >= (size_t)(kb * chainHistStride_) &&
+           longList_.n >= (size_t)long_list_entries(chainMStride_, long_chain_shift(dim_, chainMStride_)) &&
+           chainOff_.n >= (size_t)kb && chainEnd_.n >= (size_t)kb;
+}
+
+// rd_chain.hip's prefix and scatter after a histogram scan into labels; returns the shift of
+// the wave-summed chain list it wrote.
+template <typename T>
+int Engine<T>::enqueueChainOrder(int itemSize, const int* labels) {
+    const EvolveSource es = evolveSource();
+    const int64_t kb = std::max<int64_t>(keyBound_, 1);
+    const int longShift = long_chain_shift(dim_, chainMStride_);
+    const int64_t entries = long_list_entries(chainMStride_, longShift);
+    const ChainArgs<T> ca{es.P, es.stride, es.index, dItems_.get(), dSegs_.get(), dSegK_.get(), labels,
+                          chainHist_.get(), chainHistStride_, chainTotals_.get(), sortedP_.get(), chainMStride_,
+                          flags_.get(), (int)kb, chainOff_.get(), chainEnd_.get(), longList_.get(), longShift,
+                          (int)entries, dM_.get()};
+    check(launch_chain_order<T>(ca, dim_, numItems_, itemSize, (int)kb, st_), "chain order");
+    return longShift;
+}
+
 template <typename T>
 void Engine<T>::fillSentinels() {
     launch_fill_sentinel<T>(S_.get(), sStride_, dim_, dSegs_.get(), dSegK_.get(), (int)segs_.size(), st_);
@@ -582,6 +632,30 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
         return;
     }
     int* labels = log ? log->labels : labels_.get();
+    const int itemSize = std::max(64, options.scanThreads) * std::max(1, options.ppt);
+    if (chainUsable(itemSize, log, record)) {
+        // one segment: label histogram in the scan, then rd_chain.hip's prefix and scatter
+        const int64_t kb = std::max<int64_t>(keyBound_, 1);
+        ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
+                       dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
+                       nullptr, nullptr, nullptr, nullptr, chainHist_.get(), chainHistStride_, (int)kb};
+        check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
+        if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
+        // members in chain order, chain ranges and the wave-summed list, then the ordered sums
+        const int longShift = enqueueChainOrder(itemSize, labels);
+        ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
+                        dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), nullptr, dM_.get(), sortedP_.get(),
+                        chainMStride_, chainOff_.get(), chainEnd_.get(), flags_.get(),
+                        log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
+                        log ? log->moved : (record ? moved_.get() : nullptr),
+                        log ? log->movedStride : totalSlots_, longList_.get(), longShift, nullptr};
+        if (finalize)
+            ha.fin = {1, 1, dSegDone_.get(), dSegIter_.get(), capIter, flags_.get(), flags_.get() + 1,
+                      reinterpret_cast<unsigned*>(flags_.get() + 8)};
+        check(launch_chain_sum_shift<T>(ha, dim_, chainMStride_, false, st_), "chain_sum_shift");
+        if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
+        return;
+    }
     const bool rowsValid = rowSlotsVersion_ == version_ && numRowSlots_ > 0;
     const bool dense = rowsValid && denseKeys_ && compact() && !log && !record;
     ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
@@ -770,6 +844,7 @@ void Engine<T>::autotuneBatchScan() {
 
 template <typename T>
 void Engine<T>::autotuneScan() {
+    ensureChainScratch();
     const EvolveSource es = evolveSource();
     if (options.autotune && segs_.size() > 1) {
         autotuneBatchScan();
@@ -786,7 +861,7 @@ void Engine<T>::autotuneScan() {
     int64_t kTotal = 0;
     for (int k : K) kTotal += k;
     const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)),
-                                     ilog2(kTotal + 1), 0);
+                                     ilog2(kTotal + 1), options.chain && !options.tolerance ? -1 : 0);
     std::pair<int, int> best{options.ppt, options.scanThreads};
     bool cached = false;
     {
@@ -817,10 +892,21 @@ void Engine<T>::autotuneScan() {
                 options.scanThreads = cands[c][1];
                 buildWorkItems();
                 sa.items = dItems_.get();
-                check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
-                check(hipEventRecord(e0, st_), "event");
-                for (int r = 0; r < 2; ++r)
+                // the form the iterations will run: with the label histogram and the member
+                // scatter (whose cost depends on the item size) when they take the rd_chain.hip
+                // path; the ordered sums after it do not depend on the shape
+                const int itemSize = cands[c][0] * cands[c][1];
+                const bool hist = chainUsable(itemSize, nullptr, false);
+                sa.blockHist = hist ? chainHist_.get() : nullptr;
+                sa.histStride = hist ? chainHistStride_ : 0;
+                sa.histKeys = hist ? (int)std::max<int64_t>(keyBound_, 1) : 0;
+                auto pass = [&] {
                     check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
+                    if (hist) (void)enqueueChainOrder(itemSize, labels_.get());
+                };
+                pass();
+                check(hipEventRecord(e0, st_), "event");
+                for (int r = 0; r < 2; ++r) pass();
                 check(hipEventRecord(e1, st_), "event");
                 check(hipEventSynchronize(e1), "event");
                 float ms = 0;
@@ -1110,6 +1196,7 @@ int Engine<T>::evolveIteration(std::vector<int>* labels, std::vector<int>* count
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
     hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(totalSlots_)), dim3(256), 0, st_, counts_.get(), totalSlots_, 0,
                        nullptr);
+    ensureChainScratch();
     enqueueIteration(true);
     std::vector<int> moved0(ns);
     check(hipMemcpyAsync(moved0.data(), dSegMoved_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost, st_), "moved");

@@ -5,6 +5,7 @@
 #include "rd_choose.hpp"
 #include "rd_common.hpp"
 #include "rd_decimate.hpp"
+#include "rd_chain.hpp"
 #include "rd_evolve.hpp"
 #include "rd_local.hpp"
 #include "rd_sort.hpp"
@@ -99,6 +100,9 @@ struct RunOptions {
                                     // reference's serial order); segment-local ones stay exact
     long long evolveGuard = 100000; // an evolve still moving after this many iterations fails
                                     // (0 = no limit, as in the reference)
+    bool chain = true;              // one segment with at most kChainMaxKeys chosen points and
+                                    // dim <= kChainMaxDim: label histogram in the scan, member
+                                    // scatter, one lane per chain sum (rd_chain.hip)
 };
 
 template <typename T>
@@ -249,6 +253,10 @@ private:
     DevBuf<int> labels_, keys_, members_, chainOff_, chainEnd_, counts_;
     DevBuf<int> longList_;   // wave-summed chains (ShiftArgs::longList)
     DevBuf<T> sums_, sortedP_;
+    // single-segment iteration of rd_chain.hip: per-block label counts [key][block], member
+    // totals, the member row stride and the key capacity they were sized for
+    DevBuf<int> chainHist_, chainTotals_;
+    int64_t chainHistStride_ = 0, chainMStride_ = 0, chainKeyCap_ = 0;
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();
     DevBuf<unsigned char> moved_;   // [dim][totalSlots]
@@ -287,6 +295,11 @@ private:
         unsigned char* moved;
         int64_t movedStride;
     };
+    // sizes the rd_chain.hip scratch for the current segment and chosen count (outside graph
+    // capture); chainUsable: the next iteration may take that path
+    void ensureChainScratch();
+    int enqueueChainOrder(int itemSize, const int* labels);
+    bool chainUsable(int itemSize, const IterLog* log, bool record) const;
     // finalize: also the per-segment convergence bookkeeping (iteration cap capIter)
     void autotuneBatchScan();
     void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr, bool finalize = false,

@@ -79,31 +79,6 @@ __global__ __launch_bounds__(256) void gather_members_kernel(const T* __restrict
 // -0, so it is never -0 and adding +0 leaves it unchanged.  The shift of one coordinate
 // depends only on that coordinate's sum (rd/cpu/brute_force/evolve.hpp:129-179).
 // ---------------------------------------------------------------------------------
-// Mean of a chain and the shift rule of rd/cpu/brute_force/evolve.hpp:129-179 for one
-// coordinate; optional per-slot outputs.
-template <typename T>
-__device__ __forceinline__ void shift_coordinate(const ShiftArgs<T>& a, int d, int64_t slot, int seg, int cnt,
-                                                 T sum) {
-    if (a.outCounts && d == 0) a.outCounts[slot] = cnt;
-    if (a.outSums) a.outSums[d * a.sStride + slot] = sum;
-    bool moved = false;
-    if (cnt != 0) {
-        const T fc = (T)cnt;
-        constexpr T eps = FpTraits<T>::eps;
-        T mc = sum / fc;
-        T thr = T(2) * fabs(mc);
-        thr = thr * fc;
-        thr = thr * eps;
-        T& s = a.S[d * a.sStride + slot];
-        if (fabs(mc - s) > thr) {
-            s = mc;
-            moved = true;
-        }
-    }
-    if (a.outMoved) a.outMoved[d * a.movedStride + slot] = moved ? 1 : 0;
-    if (moved) a.segMoved[seg] = 1;
-}
-
 // ---------------------------------------------------------------------------------
 // 4b'. ordered sums through LDS: a group of kLanes lanes (a whole wave, or a 16-lane row)
 // sums one chain.  Tiles of kLanes * kLongPer members are loaded coalesced one tile before
@@ -358,31 +333,6 @@ __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wa
     }
     const T sum = group_ordered_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt, cnt, tile);
     if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
-}
-
-// Per-segment iteration count and convergence after an iteration, and the batch's all-done
-// flag, by one block of any size.
-__device__ __forceinline__ void finalize_segments(int numSegs, int* __restrict__ segDone, int* __restrict__ segMoved,
-                                                  int* __restrict__ segIter, int maxIter, int* __restrict__ allDone,
-                                                  int* __restrict__ totalIters) {
-    __shared__ int active;
-    if (threadIdx.x == 0) active = 0;
-    __syncthreads();
-    int mine = 0;
-    for (int s = threadIdx.x; s < numSegs; s += blockDim.x) {
-        if (segDone[s]) continue;
-        int it = segIter[s] + 1;
-        segIter[s] = it;
-        bool done = (segMoved[s] == 0) || (maxIter > 0 && it >= maxIter);
-        segMoved[s] = 0;
-        if (done) segDone[s] = 1; else ++mine;
-    }
-    if (mine) atomicAdd(&active, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (active == 0) *allDone = 1;
-        *totalIters += 1;
-    }
 }
 
 // blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains.  With

@@ -33,6 +33,12 @@ struct ScanArgs {
     // optional (compact form): the key written for slot s is denseOf[s], its index among
     // the live slots of the evolve call, so that the member sort spans only those
     const int* denseOf;
+    // optional (one segment, one block per item): every block also writes the histogram of
+    // its samples' labels, blockHist[k * histStride + item] for k < K (see rd_chain.hip), and
+    // blocks take their items in the XCD-aware order of xcd_item()
+    int* blockHist;
+    int64_t histStride;
+    int histKeys;    // keys counted: labels below min(K, histKeys)
 };
 
 template <typename T>
@@ -87,6 +93,56 @@ struct ShiftArgs {
     // reference's sample order; the mean is that sum over the count, rounded once
     int tolerance;
 };
+
+// Mean of a chain and the shift rule of rd/cpu/brute_force/evolve.hpp:129-179 for one
+// coordinate; optional per-slot outputs.
+template <typename T>
+__device__ __forceinline__ void shift_coordinate(const ShiftArgs<T>& a, int d, int64_t slot, int seg, int cnt,
+                                                 T sum) {
+    if (a.outCounts && d == 0) a.outCounts[slot] = cnt;
+    if (a.outSums) a.outSums[d * a.sStride + slot] = sum;
+    bool moved = false;
+    if (cnt != 0) {
+        const T fc = (T)cnt;
+        constexpr T eps = FpTraits<T>::eps;
+        T mc = sum / fc;
+        T thr = T(2) * fabs(mc);
+        thr = thr * fc;
+        thr = thr * eps;
+        T& s = a.S[d * a.sStride + slot];
+        if (fabs(mc - s) > thr) {
+            s = mc;
+            moved = true;
+        }
+    }
+    if (a.outMoved) a.outMoved[d * a.movedStride + slot] = moved ? 1 : 0;
+    if (moved) a.segMoved[seg] = 1;
+}
+
+// Per-segment iteration count and convergence after an iteration, and the batch's all-done
+// flag, by one block of any size.
+__device__ __forceinline__ void finalize_segments(int numSegs, int* __restrict__ segDone, int* __restrict__ segMoved,
+                                                  int* __restrict__ segIter, int maxIter, int* __restrict__ allDone,
+                                                  int* __restrict__ totalIters) {
+    __shared__ int active;
+    if (threadIdx.x == 0) active = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < numSegs; s += blockDim.x) {
+        if (segDone[s]) continue;
+        int it = segIter[s] + 1;
+        segIter[s] = it;
+        bool done = (segMoved[s] == 0) || (maxIter > 0 && it >= maxIter);
+        segMoved[s] = 0;
+        if (done) segDone[s] = 1; else ++mine;
+    }
+    if (mine) atomicAdd(&active, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (active == 0) *allDone = 1;
+        *totalIters += 1;
+    }
+}
 
 // L of the wave-summed chains for up to maxM sorted members: the smallest power of two
 // >= 2^kLongMinShift with at most kLongWaves (coordinate, list entry) waves.

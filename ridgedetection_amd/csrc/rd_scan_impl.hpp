@@ -3,7 +3,7 @@
 // (dimension x samples per lane x block size) variants compile in parallel.
 #pragma once
 
-#include "rd_evolve.hpp"
+#include "rd_chain.hpp"
 #include "rd_nearest.hpp"
 
 namespace rdhip {
@@ -11,8 +11,10 @@ namespace rdhip {
 // ---------------------------------------------------------------------------------
 // 1. nearest chosen point
 // ---------------------------------------------------------------------------------
+// lh (optional): the block's histogram of the labels below lhKeys, in LDS
 template <typename T, int DIM, int PPT, int TH>
-__device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem it) {
+__device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem it, int* lh = nullptr,
+                                          int lhKeys = 0) {
     const SegDesc sd = a.segs[it.seg];
     const int tid = threadIdx.x;
     int* __restrict__ lab = a.labels + sd.pOff;
@@ -48,7 +50,10 @@ __device__ __forceinline__ void scan_item(const ScanArgs<T>& a, const WorkItem i
     nearest_labels<T, DIM, PPT>(p, valid, K, a.S + sd.sOff, a.sStride, a.segRSqr[it.seg], label);
 #pragma unroll
     for (int j = 0; j < PPT; ++j)
-        if (valid[j]) put(idx[j], label[j] < 0 ? -1 : (int)(sd.sOff + label[j]));
+        if (valid[j]) {
+            put(idx[j], label[j] < 0 ? -1 : (int)(sd.sOff + label[j]));
+            if (lh && label[j] >= 0 && label[j] < lhKeys) atomicAdd(&lh[label[j]], 1);
+        }
 }
 
 // one block per work item (blocks past a device-side item count exit)
@@ -57,6 +62,22 @@ __global__ __launch_bounds__(TH) void evolve_scan_kernel(ScanArgs<T> a) {
     if (*a.allDone) return;
     if (a.numItemsDev && (int)blockIdx.x >= *a.numItemsDev) return;
     scan_item<T, DIM, PPT, TH>(a, a.items[blockIdx.x]);
+}
+
+// one segment: one block per item in XCD-aware order, and the histogram of the item's labels
+// (slot - sOff < K) into column b of blockHist (the member scatter of rd_chain.hip reads it)
+template <typename T, int DIM, int PPT, int TH>
+__global__ __launch_bounds__(TH) void evolve_scan_hist_kernel(ScanArgs<T> a) {
+    __shared__ int lh[kChainMaxKeys];
+    if (*a.allDone) return;
+    const int b = xcd_item(blockIdx.x, gridDim.x);
+    const WorkItem it = a.items[b];
+    const int K = min(min(a.segK[it.seg], a.histKeys), kChainMaxKeys);
+    for (int k = threadIdx.x; k < K; k += TH) lh[k] = 0;
+    __syncthreads();
+    scan_item<T, DIM, PPT, TH>(a, it, lh, K);
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += TH) a.blockHist[(int64_t)k * a.histStride + b] = lh[k];
 }
 
 // persistent blocks pulling work items from a device queue (items are ordered by
@@ -86,7 +107,10 @@ __global__ void iteration_prologue_kernel(int* __restrict__ queue, const int* __
 
 template <typename T, int DIM, int PPT, int TH>
 void launch_scan_shape(const ScanArgs<T>& a, int numItems, int persistentBlocks, hipStream_t st) {
-    if (persistentBlocks > 0) {
+    if (a.blockHist) {
+        if constexpr (DIM <= kChainMaxDim)
+            hipLaunchKernelGGL((evolve_scan_hist_kernel<T, DIM, PPT, TH>), dim3(numItems), dim3(TH), 0, st, a);
+    } else if (persistentBlocks > 0) {
         hipLaunchKernelGGL(iteration_prologue_kernel, dim3(1), dim3(1), 0, st, a.queue, a.allDone);
         hipLaunchKernelGGL((evolve_scan_persistent_kernel<T, DIM, PPT>), dim3(std::min(persistentBlocks, numItems)),
                            dim3(kScanThreads), 0, st, a, numItems);
@@ -113,6 +137,7 @@ template <typename T>
 hipError_t launch_evolve_scan_impl(const ScanArgs<T>& a, int dim, int numItems, int ppt, int threads,
                                    int persistentBlocks, hipStream_t st) {
     if (numItems <= 0) return hipSuccess;
+    if (a.blockHist && (dim > kChainMaxDim || persistentBlocks > 0)) return hipErrorInvalidValue;
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
         if (ppt == 4)
