@@ -315,7 +315,23 @@ def vjp_helix(n=200_000, iters=20, every=4, r=0.2, reps=3):
             "fwd_plus_bwd_over_fwd": (f + b) / f, "fwd_plus_bwd_over_plain_evolve": (f + b) / p}
 
 
-def fp64_helix(n=1_000_000, steps=10, warmup=3, r=0.2):
+def general_path(P, r, steps):
+    """The same iterations through the general ordering path (stable radix sort of the labels +
+    chain ranges, RDHIP_OPT_CHAIN = 0) that batches and tiles use: ms per iteration, graph
+    replay, autotuned scan shape."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN
+    with Context(P.shape[1], P.dtype) as c:
+        c.set_option(OPT_CHAIN, 0)
+        c.set_points(P)
+        c.choose(r)
+        c.evolve_fixed(r, 8)
+        t0 = time.perf_counter()
+        c.evolve_fixed(r, steps)
+        el = time.perf_counter() - t0
+    return {"ms_per_step": el * 1e3 / steps, "pair_evals_per_s": None}
+
+
+def fp64_helix(n=1_000_000, steps=40, warmup=8, r=0.2):
     """The headline workload in FP64 (same seeded helix, cast): scan kernel and whole
     iteration times, against the FP64 vector peak."""
     from ridgedetection_amd import synth
@@ -325,12 +341,14 @@ def fp64_helix(n=1_000_000, steps=10, warmup=3, r=0.2):
         c.set_points(P)
         S0, _ = c.choose(r)
         K = len(S0)
-        c.set_option(OPT_PROFILE, 1)
         c.evolve_fixed(r, warmup)
-        tb = c.timers()
         t0 = time.perf_counter()
-        c.evolve_fixed(r, steps)
+        c.evolve_fixed(r, steps)                 # graph replay, as the headline
         el = time.perf_counter() - t0
+        c.set_option(OPT_PROFILE, 1)
+        c.evolve_fixed(r, 2)
+        tb = c.timers()
+        c.evolve_fixed(r, steps)                 # one by one, HIP events around the kernels
         ta = c.timers()
     launches = max(ta["scan_launches"] - tb["scan_launches"], 1)
     scan_ms = (ta["scan_kernel_ms"] - tb["scan_kernel_ms"]) / launches
@@ -549,8 +567,10 @@ def measured_traffic(n, dtype):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the library replays captured batches of 8 iterations: 8 warm-up steps capture the graph,
+    # and 40 timed steps are 5 whole batches
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--ppt", type=int, default=0, help="samples per lane (default: autotuned)")
@@ -568,6 +588,8 @@ def main():
                     help="only run the 512-cloud batch (one mixed-dimension batch), print it and exit")
     ap.add_argument("--batch-cloud", type=int, default=-1, metavar="IDX",
                     help="only run cloud IDX of the 512-cloud batch on its own (profiling) and exit")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline iterations (profiling): no secondary measurements, no CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -605,15 +627,21 @@ def main():
     log(f"bench: K={K} chosen points")
     if args.ppt:
         ctx.set_option(OPT_PPT, args.ppt)
-    ctx.set_option(OPT_PROFILE, 1)
+    # timed region: the production form (iterations replayed from a captured graph); every
+    # library call returns with its stream synchronised
     ctx.evolve_fixed(r, args.warmup)
-    # every library call returns with its stream synchronised
-    t_before = ctx.timers()
     t0 = time.perf_counter()
     ctx.evolve_fixed(r, args.steps)
     el = time.perf_counter() - t0
-    t_after = ctx.timers()
     log(f"bench: {args.steps} iterations in {el * 1e3:.2f} ms")
+    # then the same iterations launched one by one with HIP events around the scan kernel and
+    # around each iteration on the library's stream (the roofline's kernel time)
+    ctx.set_option(OPT_PROFILE, 1)
+    ctx.evolve_fixed(r, 2)
+    t_before = ctx.timers()
+    ctx.evolve_fixed(r, args.steps)
+    t_after = ctx.timers()
+    ctx.set_option(OPT_PROFILE, 0)
 
     n = args.n
     pairs = n * K
@@ -652,11 +680,19 @@ def main():
                                   "autotuned": not args.ppt}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(n, args.dtype),
-                     "kernel": "evolve_scan_kernel", "avg_launch_ms": scan_ms,
+                     "kernel": "evolve_scan_hist_kernel" if t_after["chain_path"] else "evolve_scan_kernel",
+                     "avg_launch_ms": scan_ms,
                      "flops_per_launch": flops},
         "secondary": secondary,
     }
     ctx.close()
+    if args.headline_only:
+        print(json.dumps(result), flush=True)
+        return
+    log("bench: the same iterations through the general ordering path")
+    gp = general_path(P, r, args.steps)
+    gp["pair_evals_per_s"] = pairs * 1e3 / gp["ms_per_step"]
+    result["secondary"]["general_ordering_path"] = gp
     if dt == np.float32:
         log("bench: FP64 helix and FP32-vs-FP64 errors")
         result["secondary"]["fp64_helix_1M"] = fp64_helix(n=args.n)

@@ -99,6 +99,8 @@ typedef struct rdhip_timers {
                                 per segment and iteration, its samples x its chosen points */
     int64_t scan_persistent; /* 1: the scan's blocks pull work items from a device queue */
     int64_t vjp_checkpoint_every;  /* iterations per recorded run of the last rdhip_evolve_fwd */
+    int64_t chain_path;      /* 1: the last evolve iteration enqueued ordered its members with the
+                                single-cloud path (RDHIP_OPT_CHAIN), 0: with the general sort */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */

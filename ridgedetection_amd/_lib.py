@@ -51,7 +51,7 @@ class Timers(C.Structure):
                 ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("evolve_bytes", C.c_double),
                 ("scan_ppt", C.c_int64), ("scan_threads", C.c_int64), ("tune_ms", C.c_double),
                 ("evolve_pairs", C.c_double), ("scan_persistent", C.c_int64),
-                ("vjp_checkpoint_every", C.c_int64)]
+                ("vjp_checkpoint_every", C.c_int64), ("chain_path", C.c_int64)]
 
 
 _lib = None

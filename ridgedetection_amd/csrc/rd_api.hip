@@ -508,6 +508,7 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->evolve_pairs = e.timers.evolvePairs;
         out->scan_persistent = e.timers.scanPersistent;
         out->vjp_checkpoint_every = e.timers.vjpCheckpointEvery;
+        out->chain_path = e.timers.chainPath;
         return RDHIP_OK;
     });
 }

@@ -120,6 +120,7 @@ Engine<T>::Engine(int dim, hipStream_t stream) : dim_(dim), st_(stream) {
 
 template <typename T>
 Engine<T>::~Engine() {
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
     for (auto ev : events_) (void)hipEventDestroy(ev);
     if (decWs_.nbr) (void)hipFree(decWs_.nbr);
     if (decWs_.par) (void)hipFree(decWs_.par);
@@ -633,7 +634,8 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
     }
     int* labels = log ? log->labels : labels_.get();
     const int itemSize = std::max(64, options.scanThreads) * std::max(1, options.ppt);
-    if (chainUsable(itemSize, log, record)) {
+    timers.chainPath = chainUsable(itemSize, log, record) ? 1 : 0;
+    if (timers.chainPath) {
         // one segment: label histogram in the scan, then rd_chain.hip's prefix and scatter
         const int64_t kb = std::max<int64_t>(keyBound_, 1);
         ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
@@ -977,12 +979,15 @@ void Engine<T>::evolveFixed(long long iters) {
         if (options.profile) readEvents(nb);
         done += nb;
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     addEvolveBytes(std::vector<int>(ns, (int)std::min<long long>(iters, 0x7fffffff)));
     timers.evolveIterations += iters;
     timers.evolveMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Captures `batch` iterations into a graph.  The engine keeps one executable graph: a new
+// capture of the same shape only updates its kernel arguments (hipGraphExecUpdate), so repeated
+// evolve calls do not pay for an instantiation each; another shape replaces it.  Every caller
+// has synchronised the stream after its last launch, so the graph is idle when updated.
 template <typename T>
 bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGraphExec_t* exec) {
     *exec = nullptr;
@@ -1004,13 +1009,26 @@ bool Engine<T>::captureIterations(int batch, bool finalize, int capIter, hipGrap
         (void)hipGetLastError();
         return false;
     }
-    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (graphExec_) {
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(graphExec_, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            (void)hipGraphDestroy(g);
+            *exec = graphExec_;
+            return true;
+        }
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(graphExec_);
+        graphExec_ = nullptr;
+    }
+    e = hipGraphInstantiate(&graphExec_, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        *exec = nullptr;
+        graphExec_ = nullptr;
         (void)hipGetLastError();
         return false;
     }
+    *exec = graphExec_;
     return true;
 }
 
@@ -1148,10 +1166,7 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
         if (options.profile) readEvents(batch);
         iters = f[1];
         if (f[0]) break;
-        if (options.evolveGuard > 0 && iters >= options.evolveGuard) {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            throw std::runtime_error(guardMessage());
-        }
+        if (options.evolveGuard > 0 && iters >= options.evolveGuard) throw std::runtime_error(guardMessage());
         if (compact() && !segmentedNow_) {
             // shrink the active work list once half of it belongs to converged segments
             check(hipMemcpy(doneNow.data(), dSegDone_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "done");
@@ -1161,15 +1176,10 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
             if (2 * remaining <= activeWork_) {
                 rebuildActive(doneNow);
                 prepareRows(doneNow);
-                if (exec) {
-                    (void)hipGraphExecDestroy(exec);
-                    exec = nullptr;
-                    useGraph = captureIterations(batch, true, capIter, &exec);
-                }
+                if (exec) useGraph = captureIterations(batch, true, capIter, &exec);
             }
         }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     {
         std::vector<int> segIter(ns);
         check(hipMemcpy(segIter.data(), dSegIter_.get(), sizeof(int) * ns, hipMemcpyDeviceToHost), "iterations");

@@ -75,6 +75,7 @@ struct Timers {
     int scanPpt = 0, scanThreads = 0;   // scan shape in use (after autotuning)
     int scanPersistent = 0;             // ... and whether its blocks pull items from a queue
     long long vjpCheckpointEvery = 0;   // iterations per recorded run of the last VJP forward
+    int chainPath = 0;                  // the last iteration enqueued took the rd_chain.hip path
     double tuneMs = 0;                  // time spent timing candidate shapes
 };
 
@@ -256,6 +257,7 @@ private:
     // single-segment iteration of rd_chain.hip: per-block label counts [key][block], member
     // totals, the member row stride and the key capacity they were sized for
     DevBuf<int> chainHist_, chainTotals_;
+    hipGraphExec_t graphExec_ = nullptr;   // captured evolve iterations (captureIterations)
     int64_t chainHistStride_ = 0, chainMStride_ = 0, chainKeyCap_ = 0;
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();

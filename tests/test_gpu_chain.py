@@ -66,6 +66,7 @@ def test_every_scan_shape(oracle, shape, dtype):
     _say(f"case chain shape {shape} {np.dtype(dtype).name}")
     with _ctx(P, shape=shape) as c:
         _iterations(oracle, c, P, 0.2, 3)
+        assert c.timers()["chain_path"] == 1
 
 
 @pytest.mark.parametrize("dim", [1, 2, 3, 4])
@@ -87,6 +88,7 @@ def test_dims_and_evolve_loop(oracle, dim, dtype):
     np.testing.assert_array_equal(SR, oracle.ridge_detection(P, r1, 2 * r1)[0])
     with _ctx(P, chain=False) as g:
         np.testing.assert_array_equal(g.ridge_detection(r1, 2 * r1), SR)
+        assert g.timers()["chain_path"] == 0
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -126,3 +128,4 @@ def test_key_limit(oracle, target):
         _say(f"  K = {K}")
         assert (K <= 2048) == (target == "at_limit")
         _iterations(oracle, c, P, r, 2)
+        assert c.timers()["chain_path"] == (1 if K <= 2048 else 0)
