@@ -186,7 +186,8 @@ def batched_vs_looped(count=512, loop_every=1, seed=7, graph=True, batch_only=Fa
     t_batch = time.perf_counter() - t0
     tb = ctxs[dmax].timers()
     shape = {"samples_per_lane": tb["scan_ppt"], "threads_per_block": tb["scan_threads"],
-             "persistent": bool(tb["scan_persistent"]), "tune_ms": tb["tune_ms"]}
+             "persistent": bool(tb["scan_persistent"]), "tune_ms": tb["tune_ms"],
+             "member_ordering": "per-segment scatter" if tb["chain_path"] else "sort"}
     log(f"bench: batched {count} clouds in {t_batch:.2f} s")
     if batch_only:
         for c in ctxs.values():

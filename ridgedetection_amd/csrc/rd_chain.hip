@@ -1,20 +1,21 @@
-// rd_chain.hip -- member ordering for the evolve iteration of one segment with at most
-// kChainMaxKeys chosen points (a whole cloud), after the scan (rd_scan_impl.hpp) has written
-// every sample's label and every block's label histogram.
+// rd_chain.hip -- member ordering for evolve iterations of segments (whole clouds, the clouds
+// of a batch, tiles) with at most kChainMaxKeys chosen points each, after the scan
+// (rd_scan_impl.hpp) has written every sample's label and every block's label histogram.
 //
 // Reference semantics: rd/cpu/brute_force/evolve.hpp:32-77 (the members of every chosen point,
 // summed in sample order).
 //
-//   key_prefix     -- one wave per key: the key's row of block counts becomes its exclusive
-//                     prefix over blocks, and the key's member total.
+//   key_prefix     -- one wave per (segment, key): the key's row of block counts becomes its
+//                     exclusive prefix over the segment's blocks, and the key's member total.
 //   member_scatter -- one block per scan item (same samples, same XCD-aware order): ranks the
 //                     item's samples by key in sample order (ballot match per key bit, as in
 //                     rd_sort.hip), reads their coordinates coalesced, and writes them in chain
 //                     order through an LDS stage, so consecutive threads write consecutive
-//                     addresses of each chain run.  Chain k lies in [chainOff[k], chainEnd[k])
-//                     with chainEnd[k] the running sum of the totals rounded up to 16 bytes.
-//                     Block 0 also writes the chain ranges and the wave-summed chain list of the
-//                     ordered sums (chain_sum_shift_kernel, rd_evolve.hip), which follow.
+//                     addresses of each chain run.  A segment's chain k lies in its region at
+//                     [chainOff, chainEnd), chainEnd the running sum of the totals rounded up to
+//                     16 bytes.  Each segment's first block also writes the chain ranges and
+//                     lists the long chains for the ordered sums (chain_sum_shift_kernel,
+//                     rd_evolve.hip), which follow.
 // This replaces the histogram pass, the two-kernel histogram scan, the sort's scatter and the
 // chain-range pass of the general path (rd_sort.hip + chain bounds) for this case.
 #include "rd_chain.hpp"
@@ -49,48 +50,62 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
 }
 
 // ---------------------------------------------------------------------------------
-// per-key prefix over blocks
+// per-key prefix over a segment's blocks: one wave per (segment, key) row
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void key_prefix_kernel(int* __restrict__ hist, int64_t histStride, int nb,
-                                                         const int* __restrict__ segK, int kb,
-                                                         int* __restrict__ totals, const int* __restrict__ allDone) {
-    if (*allDone) return;
+// segment of a row: the last s with segRowOff[s] <= row
+__device__ __forceinline__ int row_segment(const int* __restrict__ rowOff, int numSegs, int row) {
+    int lo = 0, hi = numSegs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rowOff[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void key_prefix_kernel(ChainArgs<T> a) {
+    if (*a.allDone) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.longCount = 0;   // the scatter's long-chain list
     const int lane = lane_id();
-    const int k = blockIdx.x * 4 + threadIdx.x / kWave;
-    if (k >= kb) return;
-    int* __restrict__ row = hist + (int64_t)k * histStride;
+    const int row = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (row >= a.totalRows) return;
+    const int s = row_segment(a.segRowOff, a.numSegs, row);
+    const int k = row - a.segRowOff[s];
+    const SegDesc sd = a.segs[s];
+    const int nb = (sd.nP + a.itemSize - 1) / a.itemSize;
+    int* __restrict__ r = a.hist + a.segHistOff[s] + (int64_t)k * nb;
     constexpr int kPer = 16;   // one pass for up to 1024 blocks
     int carry = 0;
     for (int base = 0; base < nb; base += kWave * kPer) {
         const int i0 = base + lane * kPer;
         int v[kPer];
-        int s = 0;
+        int sum = 0;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            v[j] = i0 + j < nb ? row[i0 + j] : 0;
-            s += v[j];
+            v[j] = i0 + j < nb ? r[i0 + j] : 0;
+            sum += v[j];
         }
-        // rows of keys past the live count hold stale counts: loaded (in bounds, in parallel
-        // with the count's load) but neither rewritten nor totalled
-        if (k >= min(segK[0], kb)) {
-            if (lane == 0) totals[k] = 0;
+        // rows of converged segments and of keys past the live count hold stale counts: loaded
+        // (in bounds, in parallel with the flags' loads) but neither rewritten nor totalled
+        if (a.segDone[s] || k >= min(a.segK[s], a.segKeyCap[s])) {
+            if (lane == 0) a.totals[row] = 0;
             return;
         }
-        int incl = s;
+        int incl = sum;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const int y = __shfl_up(incl, o);
             if (lane >= o) incl += y;
         }
-        int run = carry + incl - s;
+        int run = carry + incl - sum;
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            if (i0 + j < nb) row[i0 + j] = run;
+            if (i0 + j < nb) r[i0 + j] = run;
             run += v[j];
         }
         carry += __shfl(incl, kWave - 1);
     }
-    if (lane == 0) totals[k] = carry;
+    if (lane == 0) a.totals[row] = carry;
 }
 
 // ---------------------------------------------------------------------------------
@@ -115,10 +130,17 @@ __global__ __launch_bounds__(W * kWave) void member_scatter_kernel(ChainArgs<T> 
     int* lbase = gb + keyCap;
     int* wsum = lbase + keyCap;
     if (*a.allDone) return;
-    const int K = min(a.segK[0], a.keyBound);
-    const int b = xcd_item(blockIdx.x, gridDim.x);
-    const WorkItem it = a.items[b];
-    const SegDesc sd = a.segs[it.seg];
+    const int nItems = a.numItemsDev ? *a.numItemsDev : (int)gridDim.x;
+    if ((int)blockIdx.x >= nItems) return;
+    const WorkItem it = a.items[xcd_item(blockIdx.x, nItems)];
+    const int s = it.seg;
+    if (a.segDone[s]) return;
+    const SegDesc sd = a.segs[s];
+    const int K = min(min(a.segK[s], a.segKeyCap[s]), keyCap);
+    const int col = it.start / a.itemSize;
+    const int nb = (sd.nP + a.itemSize - 1) / a.itemSize;
+    const int rowOff = a.segRowOff[s];
+    const int64_t memberOff = a.segMemberOff[s];
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
     const int kpt = keyCap / TH;
 
@@ -137,18 +159,17 @@ __global__ __launch_bounds__(W * kWave) void member_scatter_kernel(ChainArgs<T> 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) p[c][d] = a.P[d * a.pStride + n];
     }
+    const int* __restrict__ hcol = a.hist + a.segHistOff[s] + col;
     int tk[kMaxKeysPerThread], hv[kMaxKeysPerThread];
 #pragma unroll
     for (int j = 0; j < kMaxKeysPerThread; ++j) {
         const int k = tid * kpt + j;
         const bool live = j < kpt && k < K;
-        tk[j] = live ? a.totals[k] : 0;
-        hv[j] = live ? a.hist[(int64_t)k * a.histStride + b] : 0;
+        tk[j] = live ? a.totals[rowOff + k] : 0;
+        hv[j] = live ? hcol[(int64_t)k * nb] : 0;
     }
 
     for (int v = tid; v < W * keyCap; v += TH) cnt[v] = 0;
-    if (blockIdx.x == 0)
-        for (int e = tid; e < a.longEntries; e += TH) a.longList[e] = -1;
     // chain k ends at the running sum of the totals rounded up to A values
     {
         int sk[kMaxKeysPerThread];
@@ -159,26 +180,28 @@ __global__ __launch_bounds__(W * kWave) void member_scatter_kernel(ChainArgs<T> 
             sum += sk[j];
         }
         int all;
-        int end = block_excl_scan<W>(sum, wsum, all);   // (its barriers order the -1 fill above)
+        int end = block_excl_scan<W>(sum, wsum, all);
 #pragma unroll
         for (int j = 0; j < kMaxKeysPerThread; ++j) {
             const int k = tid * kpt + j;
             end += sk[j];
             if (j < kpt && k < K) {
-                const int base = end - tk[j];
-                gb[k] = base + hv[j];
-                if (blockIdx.x == 0) {
-                    a.chainOff[k] = base;
-                    a.chainEnd[k] = end;
-                    // the chain is summed by a wave when it holds a multiple of L; the first such
-                    // position names it in the wave list
-                    const int L = 1 << a.longShift;
-                    const int m0 = (base + L - 1) >> a.longShift;
-                    if (tk[j] > 0 && m0 * L < end) a.longList[m0] = k;
+                const int64_t base = memberOff + end - tk[j];
+                gb[k] = (int)(base + hv[j]);
+                if (col == 0) {
+                    const int64_t slot = sd.sOff + k;
+                    a.chainOff[slot] = (int)base;
+                    a.chainEnd[slot] = (int)(memberOff + end);
+                    // long chains take a place in the wave list (which place does not matter:
+                    // a chain's sum is the same on either path)
+                    if (tk[j] >= a.longMin) {
+                        const int idx = atomicAdd(a.longCount, 1);
+                        a.longIdx[slot] = idx;
+                        if (idx < a.longCap) a.longList[idx] = (int)slot;
+                    }
                 }
             }
         }
-        if (blockIdx.x == 0 && tid == 0) *a.mCount = all;
     }
     __syncthreads();
 
@@ -280,9 +303,9 @@ void launch_scatter(const ChainArgs<T>& a, int numItems, int keyCap, hipStream_t
 }
 
 template <typename T, int DIM>
-bool launch_chain_dim(const ChainArgs<T>& a, int numItems, int itemSize, int kb, hipStream_t st) {
-    const int keyCap = round_up_int(std::max(kb, 1), 256);
-    switch (itemSize) {
+bool launch_chain_dim(const ChainArgs<T>& a, int numItems, int keyCapMax, hipStream_t st) {
+    const int keyCap = round_up_int(std::max(keyCapMax, 1), 256);
+    switch (a.itemSize) {
         case 128: launch_scatter<T, DIM, 2, 1>(a, numItems, keyCap, st); break;
         case 256: launch_scatter<T, DIM, 4, 1>(a, numItems, keyCap, st); break;
         case 512: launch_scatter<T, DIM, 4, 2>(a, numItems, keyCap, st); break;
@@ -295,29 +318,28 @@ bool launch_chain_dim(const ChainArgs<T>& a, int numItems, int itemSize, int kb,
 
 }  // namespace
 
-bool chain_iteration_supported(int dim, int itemSize, int64_t kb) {
+bool chain_iteration_supported(int dim, int itemSize, int64_t keyCapMax) {
     const bool size = itemSize == 128 || itemSize == 256 || itemSize == 512 || itemSize == 1024 || itemSize == 2048;
-    return dim >= 1 && dim <= kChainMaxDim && size && kb >= 1 && kb <= kChainMaxKeys;
+    return dim >= 1 && dim <= kChainMaxDim && size && keyCapMax >= 1 && keyCapMax <= kChainMaxKeys;
 }
 
 template <typename T>
-hipError_t launch_chain_order(const ChainArgs<T>& a, int dim, int numItems, int itemSize, int kb, hipStream_t st) {
-    if (!chain_iteration_supported(dim, itemSize, kb) || a.keyBound != kb) return hipErrorInvalidValue;
-    if (numItems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(key_prefix_kernel, dim3((unsigned)((kb + 3) / 4)), dim3(256), 0, st, a.hist, a.histStride,
-                       numItems, a.segK, kb, a.totals, a.allDone);
+hipError_t launch_chain_order(const ChainArgs<T>& a, int dim, int numItems, int keyCapMax, hipStream_t st) {
+    if (!chain_iteration_supported(dim, a.itemSize, keyCapMax)) return hipErrorInvalidValue;
+    if (numItems <= 0 || a.totalRows <= 0) return hipSuccess;
+    hipLaunchKernelGGL((key_prefix_kernel<T>), dim3((unsigned)((a.totalRows + 3) / 4)), dim3(256), 0, st, a);
     bool ok = false;
     switch (dim) {
-        case 1: ok = launch_chain_dim<T, 1>(a, numItems, itemSize, kb, st); break;
-        case 2: ok = launch_chain_dim<T, 2>(a, numItems, itemSize, kb, st); break;
-        case 3: ok = launch_chain_dim<T, 3>(a, numItems, itemSize, kb, st); break;
-        case 4: ok = launch_chain_dim<T, 4>(a, numItems, itemSize, kb, st); break;
+        case 1: ok = launch_chain_dim<T, 1>(a, numItems, keyCapMax, st); break;
+        case 2: ok = launch_chain_dim<T, 2>(a, numItems, keyCapMax, st); break;
+        case 3: ok = launch_chain_dim<T, 3>(a, numItems, keyCapMax, st); break;
+        case 4: ok = launch_chain_dim<T, 4>(a, numItems, keyCapMax, st); break;
         default: break;
     }
     return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 
-template hipError_t launch_chain_order<float>(const ChainArgs<float>&, int, int, int, int, hipStream_t);
-template hipError_t launch_chain_order<double>(const ChainArgs<double>&, int, int, int, int, hipStream_t);
+template hipError_t launch_chain_order<float>(const ChainArgs<float>&, int, int, int, hipStream_t);
+template hipError_t launch_chain_order<double>(const ChainArgs<double>&, int, int, int, hipStream_t);
 
 }  // namespace rdhip

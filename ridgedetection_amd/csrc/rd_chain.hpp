@@ -1,7 +1,7 @@
-// rd_chain.hpp -- member ordering for the evolve iteration of one segment with up to
-// kChainMaxKeys chosen points (a whole cloud) without a separate histogram pass: the scan
-// counts labels per block, one kernel turns the counts into per-key prefixes and one scatter
-// writes the member coordinates in chain order.  Kernels in rd_chain.hip.
+// rd_chain.hpp -- member ordering for evolve iterations without a sort, for segments (whole
+// clouds of a batch, tiles) with up to kChainMaxKeys chosen points each: the scan counts labels
+// per block, one kernel turns the counts into per-key prefixes over the segment's blocks and
+// one scatter writes the member coordinates in chain order.  Kernels in rd_chain.hip.
 #pragma once
 
 #include "rd_evolve.hpp"
@@ -18,50 +18,56 @@ constexpr int kChainMaxDim = 4;
 template <typename T>
 constexpr int kChainAlign = 16 / (int)sizeof(T);
 
-// Stride of the member coordinate rows for n samples and at most kCap keys (the alignment
-// gaps add fewer than 4 values per key).
-inline int64_t chain_member_stride(int64_t n, int64_t kCap) {
-    const int64_t m = n + 4 * kCap + 64;
-    return (m + 63) / 64 * 64;
-}
+// Positions segment s's chains may take: its samples plus the alignment gaps (fewer than 4
+// values per key), rounded to 64.
+inline int64_t chain_region(int64_t nP, int64_t keys) { return (nP + 4 * keys + 64 + 63) / 64 * 64; }
 
+// Per-segment layout (host-built, device arrays): segment s orders keys k < min(segK[s],
+// segKeyCap[s]); its per-block counts are the table hist[segHistOff[s] + k * nb_s + col] with
+// nb_s = ceil(nP_s / itemSize) blocks (col = item start / itemSize); its per-key totals are
+// totals[segRowOff[s] + k]; its chains take the positions [segMemberOff[s], segMemberOff[s + 1]).
 template <typename T>
 struct ChainArgs {
-    // samples of the one segment
     const T* P;
     int64_t pStride;
     const int* pIndex;
-    const WorkItem* items;   // the scan's items, in sample order; item b's samples are block b's
+    const WorkItem* items;      // the scan's items (any order); item b covers block b
+    const int* numItemsDev;     // optional: the items in use (else one per block)
     const SegDesc* segs;
     const int* segK;
-    const int* labels;       // per segment position: chosen slot, or -1
-    // hist[k * histStride + b]: the scan's per-block label counts; key_prefix turns every
-    // row into its exclusive prefix over blocks
+    const int* segDone;
+    int numSegs;
+    const int* labels;          // per position: chosen slot, or -1
     int* hist;
-    int64_t histStride;
-    int* totals;             // members per key
-    T* sortedP;              // member coordinates, SoA rows of mStride, chain layout above
+    const int64_t* segHistOff;
+    const int* segKeyCap;
+    const int* segRowOff;       // numSegs + 1 entries
+    int itemSize;
+    int totalRows;
+    int* totals;
+    T* sortedP;                 // member coordinates, SoA rows of mStride
     int64_t mStride;
+    const int64_t* segMemberOff;   // numSegs + 1 entries
     const int* allDone;
-    int keyBound;            // keys handled: k < min(segK[0], keyBound) (the buffers' capacity)
-    // written by the scatter's block 0 for the ordered sums (ShiftArgs of rd_evolve.hpp):
-    // chain ranges, the wave-summed chain list over longEntries multiples of 2^longShift, and
-    // the end of the last chain
+    // written by each segment's first block for the ordered sums (ShiftArgs of rd_evolve.hpp):
+    // chain ranges per slot, and for every chain of at least longMin members an index from
+    // *longCount (reset by the prefix pass) into longIdx[slot], the first longCap of them listed
+    // in longList for the wave-summed path
     int* chainOff;
     int* chainEnd;
     int* longList;
-    int longShift;
-    int longEntries;
-    int64_t* mCount;
+    int* longCount;
+    int* longIdx;
+    int longMin;
+    int longCap;
 };
 
-// Launches the per-key prefix and the member scatter, for a scan that wrote labels and
-// blockHist over numItems items of itemSize samples.  kb (== a.keyBound) bounds the number of
-// keys; the device count segK[0] is authoritative below it.
+// Launches the per-key prefix and the member scatter after a histogram scan over numItems
+// blocks (the items in use may be fewer: a.numItemsDev); keyCapMax bounds segKeyCap.
 template <typename T>
-hipError_t launch_chain_order(const ChainArgs<T>& a, int dim, int numItems, int itemSize, int kb, hipStream_t st);
+hipError_t launch_chain_order(const ChainArgs<T>& a, int dim, int numItems, int keyCapMax, hipStream_t st);
 
 // True when launch_chain_order supports this shape.
-bool chain_iteration_supported(int dim, int itemSize, int64_t kb);
+bool chain_iteration_supported(int dim, int itemSize, int64_t keyCapMax);
 
 }  // namespace rdhip

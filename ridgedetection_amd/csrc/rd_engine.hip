@@ -275,6 +275,7 @@ void Engine<T>::buildWorkItems() {
     std::vector<WorkItem>& items = items_;
     items.clear();
     const int per = std::max(64, options.scanThreads) * std::max(1, options.ppt);
+    itemsPer_ = per;
     for (int s = 0; s < (int)segs_.size(); ++s)
         for (int st = 0; st < segs_[s].nP; st += per) items.push_back({s, st, std::min(segs_[s].nP, st + per), 0});
     numItems_ = (int)items.size();
@@ -422,54 +423,81 @@ void Engine<T>::ensureEvolveScratch() {
     decI64_.reserve(2);
 }
 
-// The histogram stride covers the smallest scan item (128 samples), so every scan shape fits;
-// the member rows leave room for the chains' 16-byte end alignment and the sums' prefetch.
+// Layout of the rd_chain.hip member ordering for the current segments and chosen counts: per
+// segment its key capacity (its live count), its count table sized for the smallest scan item
+// (128 samples, so every scan shape fits), its rows of key totals and its member positions.
 template <typename T>
 void Engine<T>::ensureChainScratch() {
-    if (!options.chain || segs_.size() != 1 || dim_ > kChainMaxDim) return;
-    const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    if (kb > kChainMaxKeys) return;
-    const int64_t hs = round_up((totalP_ + 127) / 128, 64);
-    const int64_t ms = chain_member_stride(totalP_, kb);
-    chainHist_.reserve((size_t)(kb * hs));
-    chainTotals_.reserve((size_t)kb);
-    if (sortedP_.n < (size_t)(ms * dim_)) sortedP_.reserve((size_t)(ms * dim_));
-    const int64_t entries = long_list_entries(ms, long_chain_shift(dim_, ms));
+    chainReady_ = false;
+    const int ns = (int)segs_.size();
+    if (!options.chain || ns == 0 || dim_ > kChainMaxDim) return;
+    const std::vector<int> K = countsHost();
+    std::vector<int> cap(ns), rowOff(ns + 1);
+    std::vector<int64_t> histOff(ns), memOff(ns + 1);
+    int rows = 0, capMax = 0;
+    int64_t h = 0, m = 0;
+    for (int s = 0; s < ns; ++s) {
+        cap[s] = std::max(K[s], 0);
+        capMax = std::max(capMax, cap[s]);
+        rowOff[s] = rows;
+        rows += cap[s];
+        histOff[s] = h;
+        h += (int64_t)cap[s] * ((segs_[s].nP + 127) / 128);
+        memOff[s] = m;
+        m += chain_region(segs_[s].nP, cap[s]);
+    }
+    rowOff[ns] = rows;
+    memOff[ns] = m;
+    if (capMax > kChainMaxKeys || rows == 0 || m >= (int64_t(1) << 31)) return;
+    chainSegKeyCap_.reserve(ns);
+    chainSegRowOff_.reserve(ns + 1);
+    chainSegHistOff_.reserve(ns);
+    chainSegMemberOff_.reserve(ns + 1);
+    chainTotalPos_.reserve(1);
+    chainHist_.reserve((size_t)std::max<int64_t>(h, 1));
+    chainTotals_.reserve((size_t)rows);
+    if (sortedP_.n < (size_t)(m * dim_)) sortedP_.reserve((size_t)(m * dim_));
+    const int64_t entries = kLongWaves;   // the length-based wave list (at most kLongWaves / dim used)
     if (longList_.n < (size_t)entries) longList_.reserve((size_t)entries);
-    chainHistStride_ = hs;
-    chainMStride_ = ms;
-    chainKeyCap_ = kb;
+    chainLongIdx_.reserve((size_t)std::max<int64_t>(totalSlots_, 1));
+    chainLongCount_.reserve(1);
+    check(hipMemcpyAsync(chainSegKeyCap_.get(), cap.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "chain");
+    check(hipMemcpyAsync(chainSegRowOff_.get(), rowOff.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, st_),
+          "chain");
+    check(hipMemcpyAsync(chainSegHistOff_.get(), histOff.data(), sizeof(int64_t) * ns, hipMemcpyHostToDevice, st_),
+          "chain");
+    check(hipMemcpyAsync(chainSegMemberOff_.get(), memOff.data(), sizeof(int64_t) * (ns + 1), hipMemcpyHostToDevice,
+                         st_),
+          "chain");
+    check(hipMemcpyAsync(chainTotalPos_.get(), &m, sizeof(int64_t), hipMemcpyHostToDevice, st_), "chain");
+    check(hipStreamSynchronize(st_), "chain");   // the host vectors above are released on return
+    chainMStride_ = m;
+    chainTotalRows_ = rows;
+    chainKeyCapMax_ = std::max(capMax, 1);
+    chainEpoch_ = countsEpoch_;
+    chainReady_ = true;
 }
 
 template <typename T>
-bool Engine<T>::chainUsable(int itemSize, const IterLog*, bool) const {
-    if (!options.chain || options.tolerance || segs_.size() != 1 || segmentedNow_) return false;
-    const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    if (!chain_iteration_supported(dim_, itemSize, kb) || kb > chainKeyCap_) return false;
-    // the histogram prefix orders blocks by item: items must follow the samples
-    if ((int)items_.size() != numItems_) return false;
-    for (size_t i = 0; i < items_.size(); ++i)
-        if (items_[i].seg != 0 || items_[i].start != (int)i * itemSize) return false;
-    return chainHistStride_ >= numItems_ && chainMStride_ >= chain_member_stride(totalP_, kb) &&
-           sortedP_.n >= (size_t)(chainMStride_ * dim_) && chainHist_.n >= (size_t)(kb * chainHistStride_) &&
-           longList_.n >= (size_t)long_list_entries(chainMStride_, long_chain_shift(dim_, chainMStride_)) &&
-           chainOff_.n >= (size_t)kb && chainEnd_.n >= (size_t)kb;
+bool Engine<T>::chainUsable(int itemSize) const {
+    if (!chainReady_ || chainEpoch_ != countsEpoch_ || !options.chain || options.tolerance || segmentedNow_)
+        return false;
+    // items of itemSize samples from the starts of their segments (the count tables' columns)
+    return itemsPer_ == itemSize && chain_iteration_supported(dim_, itemSize, chainKeyCapMax_);
 }
 
-// rd_chain.hip's prefix and scatter after a histogram scan into labels; returns the shift of
-// the wave-summed chain list it wrote.
+// rd_chain.hip's prefix and scatter after a histogram scan into labels.  Chains of at least
+// 2^kLongMinShift members are listed for the wave-summed sums, up to kLongWaves waves.
 template <typename T>
-int Engine<T>::enqueueChainOrder(int itemSize, const int* labels) {
+void Engine<T>::enqueueChainOrder(int itemSize, const int* labels) {
     const EvolveSource es = evolveSource();
-    const int64_t kb = std::max<int64_t>(keyBound_, 1);
-    const int longShift = long_chain_shift(dim_, chainMStride_);
-    const int64_t entries = long_list_entries(chainMStride_, longShift);
-    const ChainArgs<T> ca{es.P, es.stride, es.index, dItems_.get(), dSegs_.get(), dSegK_.get(), labels,
-                          chainHist_.get(), chainHistStride_, chainTotals_.get(), sortedP_.get(), chainMStride_,
-                          flags_.get(), (int)kb, chainOff_.get(), chainEnd_.get(), longList_.get(), longShift,
-                          (int)entries, dM_.get()};
-    check(launch_chain_order<T>(ca, dim_, numItems_, itemSize, (int)kb, st_), "chain order");
-    return longShift;
+    const ChainArgs<T> ca{es.P, es.stride, es.index, dItems_.get(), compact() ? flags_.get() + 4 : nullptr,
+                          dSegs_.get(), dSegK_.get(), dSegDone_.get(), (int)segs_.size(), labels, chainHist_.get(),
+                          chainSegHistOff_.get(), chainSegKeyCap_.get(), chainSegRowOff_.get(), itemSize,
+                          chainTotalRows_, chainTotals_.get(), sortedP_.get(), chainMStride_,
+                          chainSegMemberOff_.get(), flags_.get(), chainOff_.get(), chainEnd_.get(), longList_.get(),
+                          chainLongCount_.get(), chainLongIdx_.get(), 1 << kLongMinShift, kLongWaves / dim_};
+    check(launch_chain_order<T>(ca, dim_, numItems_, chainKeyCapMax_, st_), "chain order");
 }
 
 template <typename T>
@@ -536,6 +564,7 @@ void Engine<T>::updateKeyBound() {
     for (size_t s = 0; s < K.size(); ++s)
         if (K[s] > 0) kb = std::max<int64_t>(kb, segs_[s].sOff + K[s]);
     keyBound_ = kb;
+    ++countsEpoch_;
 }
 
 // ------------------------------------------------------------------------------------
@@ -634,27 +663,40 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
     }
     int* labels = log ? log->labels : labels_.get();
     const int itemSize = std::max(64, options.scanThreads) * std::max(1, options.ppt);
-    timers.chainPath = chainUsable(itemSize, log, record) ? 1 : 0;
+    timers.chainPath = chainUsable(itemSize) ? 1 : 0;
     if (timers.chainPath) {
-        // one segment: label histogram in the scan, then rd_chain.hip's prefix and scatter
-        const int64_t kb = std::max<int64_t>(keyBound_, 1);
+        // label histogram in the scan (one block per item in use), rd_chain.hip's prefix and
+        // scatter, then the ordered sums over the chain layout
         ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(), dSegK_.get(),
                        dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
-                       nullptr, nullptr, nullptr, nullptr, chainHist_.get(), chainHistStride_, (int)kb};
+                       nullptr, nullptr, compact() ? flags_.get() + 4 : nullptr, nullptr, chainHist_.get(),
+                       chainSegHistOff_.get(), chainSegKeyCap_.get(), itemSize};
         check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
         if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
-        // members in chain order, chain ranges and the wave-summed list, then the ordered sums
-        const int longShift = enqueueChainOrder(itemSize, labels);
-        ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
-                        dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), nullptr, dM_.get(), sortedP_.get(),
-                        chainMStride_, chainOff_.get(), chainEnd_.get(), flags_.get(),
+        enqueueChainOrder(itemSize, labels);
+        // rows over the live slots of the evolve call (batches), else every slot below the bound
+        const bool rowsValid = rowSlotsVersion_ == version_ && numRowSlots_ > 0;
+        ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_,
+                        rowsValid ? numRowSlots_ : std::max<int64_t>(keyBound_, 1), dSlotSeg_.get(), dSegs_.get(),
+                        dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), nullptr, chainTotalPos_.get(),
+                        sortedP_.get(), chainMStride_, chainOff_.get(), chainEnd_.get(), flags_.get(),
                         log ? log->counts : (record ? counts_.get() : nullptr), record ? sums_.get() : nullptr,
                         log ? log->moved : (record ? moved_.get() : nullptr),
-                        log ? log->movedStride : totalSlots_, longList_.get(), longShift, nullptr};
-        if (finalize)
+                        log ? log->movedStride : totalSlots_, longList_.get(), 0,
+                        rowsValid ? rowSlots_.get() : nullptr};
+        ha.longCount = chainLongCount_.get();
+        ha.longIdx = chainLongIdx_.get();
+        ha.longMin = 1 << kLongMinShift;
+        ha.longCap = kLongWaves / dim_;
+        const bool fuseFinalize = finalize && segs_.size() == 1;
+        if (fuseFinalize)
             ha.fin = {1, 1, dSegDone_.get(), dSegIter_.get(), capIter, flags_.get(), flags_.get() + 1,
                       reinterpret_cast<unsigned*>(flags_.get() + 8)};
         check(launch_chain_sum_shift<T>(ha, dim_, chainMStride_, false, st_), "chain_sum_shift");
+        if (finalize && !fuseFinalize)
+            check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(),
+                                         capIter, flags_.get(), flags_.get() + 1, st_),
+                  "finalize");
         if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
         return;
     }
@@ -898,13 +940,14 @@ void Engine<T>::autotuneScan() {
                 // scatter (whose cost depends on the item size) when they take the rd_chain.hip
                 // path; the ordered sums after it do not depend on the shape
                 const int itemSize = cands[c][0] * cands[c][1];
-                const bool hist = chainUsable(itemSize, nullptr, false);
+                const bool hist = chainUsable(itemSize);
                 sa.blockHist = hist ? chainHist_.get() : nullptr;
-                sa.histStride = hist ? chainHistStride_ : 0;
-                sa.histKeys = hist ? (int)std::max<int64_t>(keyBound_, 1) : 0;
+                sa.histSegOff = hist ? chainSegHistOff_.get() : nullptr;
+                sa.histKeyCap = hist ? chainSegKeyCap_.get() : nullptr;
+                sa.histItem = hist ? itemSize : 0;
                 auto pass = [&] {
                     check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
-                    if (hist) (void)enqueueChainOrder(itemSize, labels_.get());
+                    if (hist) enqueueChainOrder(itemSize, labels_.get());
                 };
                 pass();
                 check(hipEventRecord(e0, st_), "event");

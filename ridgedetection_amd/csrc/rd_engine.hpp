@@ -101,9 +101,9 @@ struct RunOptions {
                                     // reference's serial order); segment-local ones stay exact
     long long evolveGuard = 100000; // an evolve still moving after this many iterations fails
                                     // (0 = no limit, as in the reference)
-    bool chain = true;              // one segment with at most kChainMaxKeys chosen points and
-                                    // dim <= kChainMaxDim: label histogram in the scan, member
-                                    // scatter, one lane per chain sum (rd_chain.hip)
+    bool chain = true;              // segments of at most kChainMaxKeys chosen points, dim <=
+                                    // kChainMaxDim: label histogram in the scan and one member
+                                    // scatter instead of the sort (rd_chain.hip)
 };
 
 template <typename T>
@@ -254,11 +254,16 @@ private:
     DevBuf<int> labels_, keys_, members_, chainOff_, chainEnd_, counts_;
     DevBuf<int> longList_;   // wave-summed chains (ShiftArgs::longList)
     DevBuf<T> sums_, sortedP_;
-    // single-segment iteration of rd_chain.hip: per-block label counts [key][block], member
-    // totals, the member row stride and the key capacity they were sized for
-    DevBuf<int> chainHist_, chainTotals_;
     hipGraphExec_t graphExec_ = nullptr;   // captured evolve iterations (captureIterations)
-    int64_t chainHistStride_ = 0, chainMStride_ = 0, chainKeyCap_ = 0;
+    // member ordering of rd_chain.hip: per-segment count tables, key totals and the layout
+    // arrays of ChainArgs, sized for the chosen counts of countsEpoch_ == chainEpoch_
+    DevBuf<int> chainHist_, chainTotals_, chainSegKeyCap_, chainSegRowOff_, chainLongIdx_, chainLongCount_;
+    DevBuf<int64_t> chainSegHistOff_, chainSegMemberOff_, chainTotalPos_;
+    int64_t chainMStride_ = 0;
+    int chainTotalRows_ = 0, chainKeyCapMax_ = 0;
+    bool chainReady_ = false;
+    long long chainEpoch_ = -1, countsEpoch_ = 0;   // countsEpoch_: bumped when chosen counts change
+    int itemsPer_ = 0;                              // samples per work item of buildWorkItems
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();
     DevBuf<unsigned char> moved_;   // [dim][totalSlots]
@@ -297,11 +302,11 @@ private:
         unsigned char* moved;
         int64_t movedStride;
     };
-    // sizes the rd_chain.hip scratch for the current segment and chosen count (outside graph
+    // sizes the rd_chain.hip scratch for the current segments and chosen counts (outside graph
     // capture); chainUsable: the next iteration may take that path
     void ensureChainScratch();
-    int enqueueChainOrder(int itemSize, const int* labels);
-    bool chainUsable(int itemSize, const IterLog* log, bool record) const;
+    void enqueueChainOrder(int itemSize, const int* labels);
+    bool chainUsable(int itemSize) const;
     // finalize: also the per-segment convergence bookkeeping (iteration cap capIter)
     void autotuneBatchScan();
     void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr, bool finalize = false,

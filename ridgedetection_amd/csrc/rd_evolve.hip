@@ -286,9 +286,14 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
         if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
             off = a.chainOff[chain];
             cnt = a.chainEnd[chain] - off;
-            // a chain holding a multiple of L is summed by a wave (chain_sum_wave)
+            // a chain holding a multiple of L (or, by length, one of the listed long chains) is
+            // summed by a wave (chain_sum_wave)
             const int64_t s0 = off, e1 = (int64_t)off + cnt - 1;
-            if (a.longList && cnt > 0 && (e1 >> a.longShift) >= ((s0 + (int64_t(1) << a.longShift) - 1) >> a.longShift)) {
+            const bool waveSummed =
+                a.longCount ? (cnt >= a.longMin && a.longIdx[chain] < a.longCap)
+                            : (a.longList && cnt > 0 &&
+                               (e1 >> a.longShift) >= ((s0 + (int64_t(1) << a.longShift) - 1) >> a.longShift));
+            if (waveSummed) {
                 seg = -1;
                 cnt = 0;
             }
@@ -318,7 +323,7 @@ template <typename T, int DIM>
 __device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wave, T* tile) {
     const int64_t e = wave / DIM;
     const int d = (int)(wave - e * DIM);
-    if ((e << a.longShift) >= *a.mPtr) return;
+    if (a.longCount ? e >= min(*a.longCount, a.longCap) : (e << a.longShift) >= *a.mPtr) return;
     const int key = a.longList[e];
     if (key < 0) return;
     const int slot = a.denseChains ? a.rowSlots[key] : key;
@@ -396,7 +401,8 @@ template <typename T>
 hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, bool gather, hipStream_t st) {
     const int64_t rowBlocks = ((int64_t)dim * a.totalSlots + 15) / 16;
     if (rowBlocks <= 0) return hipSuccess;
-    const int64_t waves = a.longList ? (int64_t)dim * long_list_entries(maxM, a.longShift) : 0;
+    const int64_t waves = a.longCount ? (int64_t)dim * a.longCap
+                                      : a.longList ? (int64_t)dim * long_list_entries(maxM, a.longShift) : 0;
     const int64_t blocks = rowBlocks + (waves + 256 / kWave - 1) / (256 / kWave);
     const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;

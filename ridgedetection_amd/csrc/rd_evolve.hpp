@@ -33,12 +33,14 @@ struct ScanArgs {
     // optional (compact form): the key written for slot s is denseOf[s], its index among
     // the live slots of the evolve call, so that the member sort spans only those
     const int* denseOf;
-    // optional (one segment, one block per item): every block also writes the histogram of
-    // its samples' labels, blockHist[k * histStride + item] for k < K (see rd_chain.hip), and
-    // blocks take their items in the XCD-aware order of xcd_item()
+    // optional (one block per item): every block also writes the histogram of its item's
+    // labels below min(K, histKeyCap[seg]) into its segment's table,
+    // blockHist[histSegOff[seg] + k * nb + start / histItem] with nb = ceil(nP / histItem)
+    // (rd_chain.hpp), and blocks take the items in use in the XCD-aware order of xcd_item()
     int* blockHist;
-    int64_t histStride;
-    int histKeys;    // keys counted: labels below min(K, histKeys)
+    const int64_t* histSegOff;
+    const int* histKeyCap;
+    int histItem;
 };
 
 template <typename T>
@@ -92,6 +94,13 @@ struct ShiftArgs {
     // float data, double-double for double data) in a fixed order, instead of serially in the
     // reference's sample order; the mean is that sum over the count, rounded once
     int tolerance;
+    // optional (member layouts of rd_chain.hip): the wave list by length instead of by position
+    // -- every chain of at least longMin members took an index from *longCount (longIdx[chain]);
+    // the first longCap of them are summed by waves (longList[index] = chain), the rest by rows
+    const int* longCount;
+    const int* longIdx;
+    int longMin;
+    int longCap;
 };
 
 // Mean of a chain and the shift rule of rd/cpu/brute_force/evolve.hpp:129-179 for one

@@ -64,20 +64,23 @@ __global__ __launch_bounds__(TH) void evolve_scan_kernel(ScanArgs<T> a) {
     scan_item<T, DIM, PPT, TH>(a, a.items[blockIdx.x]);
 }
 
-// one segment: one block per item in XCD-aware order, and the histogram of the item's labels
-// (slot - sOff < K) into column b of blockHist (the member scatter of rd_chain.hip reads it)
+// one block per item in use, in XCD-aware order, and the histogram of the item's labels
+// (slot - sOff below the segment's key capacity) into its segment's count table (rd_chain.hpp)
 template <typename T, int DIM, int PPT, int TH>
 __global__ __launch_bounds__(TH) void evolve_scan_hist_kernel(ScanArgs<T> a) {
     __shared__ int lh[kChainMaxKeys];
     if (*a.allDone) return;
-    const int b = xcd_item(blockIdx.x, gridDim.x);
-    const WorkItem it = a.items[b];
-    const int K = min(min(a.segK[it.seg], a.histKeys), kChainMaxKeys);
+    const int n = a.numItemsDev ? *a.numItemsDev : (int)gridDim.x;
+    if ((int)blockIdx.x >= n) return;
+    const WorkItem it = a.items[xcd_item(blockIdx.x, n)];
+    const int K = min(min(a.segK[it.seg], a.histKeyCap[it.seg]), kChainMaxKeys);
     for (int k = threadIdx.x; k < K; k += TH) lh[k] = 0;
     __syncthreads();
     scan_item<T, DIM, PPT, TH>(a, it, lh, K);
     __syncthreads();
-    for (int k = threadIdx.x; k < K; k += TH) a.blockHist[(int64_t)k * a.histStride + b] = lh[k];
+    const int nb = (a.segs[it.seg].nP + a.histItem - 1) / a.histItem;
+    int* __restrict__ col = a.blockHist + a.histSegOff[it.seg] + it.start / a.histItem;
+    for (int k = threadIdx.x; k < K; k += TH) col[(int64_t)k * nb] = lh[k];
 }
 
 // persistent blocks pulling work items from a device queue (items are ordered by

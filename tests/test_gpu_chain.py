@@ -129,3 +129,54 @@ def test_key_limit(oracle, target):
         assert (K <= 2048) == (target == "at_limit")
         _iterations(oracle, c, P, r, 2)
         assert c.timers()["chain_path"] == (1 if K <= 2048 else 0)
+
+
+def _batch_clouds(dim, count, seed):
+    g = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(count):
+        n = int(np.exp(g.uniform(np.log(2000), np.log(40000))))
+        P = synth.helix3d(n, seed=seed + i, sigma=0.05)[:, :dim] if i % 2 else synth.segment_chain(n, dim, seed=seed + i, sigma=0.03)
+        ext = float(np.max(P.max(0) - P.min(0)))
+        r1 = float(g.uniform(0.02, 0.06)) * ext
+        out.append((P.astype(np.float32), r1, 2 * r1))
+    return out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batches_take_the_path(oracle, dim):
+    """Batch-wide iterations of many clouds (the segment-local path off, so every cloud iterates
+    batch-wide) order members per cloud without the sort: equal to the general path and to the
+    oracle's single-cloud runs."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN, OPT_LOCAL
+    clouds = _batch_clouds(dim, 16, 500 + dim)
+    _say(f"case chain batch d={dim} clouds={len(clouds)}")
+    res = {}
+    for chain in (1, 0):
+        with Context(dim, np.float32) as c:
+            c.set_option(OPT_CHAIN, chain)
+            c.set_option(OPT_LOCAL, 0)
+            res[chain] = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
+            assert c.timers()["chain_path"] == chain
+    for (P, r1, r2), S, G in zip(clouds, res[1], res[0]):
+        np.testing.assert_array_equal(S, G)
+        np.testing.assert_array_equal(S, oracle.ridge_detection(P, r1, r2)[0])
+
+
+@pytest.mark.parametrize("alg", [6, 1, 5])
+def test_tiles_take_the_path(oracle, alg):
+    """Tiled runs (leaves with neighbour samples, indexed positions) through the path: equal to
+    the general path and to the oracle."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN, OPT_LOCAL
+    P = synth.helix3d(30000, seed=33)
+    _say(f"case chain tiled alg={alg}")
+    res = {}
+    for chain in (1, 0):
+        with Context(3, np.float32) as c:
+            c.set_option(OPT_CHAIN, chain)
+            c.set_option(OPT_LOCAL, 0)
+            c.set_points(P)
+            res[chain] = c.tiled(alg, 0.2, 0.4, 3000, [4, 4, 4])
+            assert c.timers()["chain_path"] == chain
+    np.testing.assert_array_equal(res[1], res[0])
+    np.testing.assert_array_equal(res[1], oracle.tiled(alg, P, 0.2, 0.4, 3000, [4, 4, 4]))
