@@ -156,11 +156,13 @@ enum rdhip_option {
                                       the mean of that sum -- no serial floor on long chains,
                                       deterministic, but means differ from the reference's by its
                                       own rounding of the serial sum */
-    RDHIP_OPT_CHAIN = 16           /* 1 (default): iterations of one cloud with at most 2048
-                                      chosen points in at most 4 dimensions count the labels in the
-                                      scan, order the members with one scatter and sum every
-                                      (chosen point, coordinate) chain in its own lane; 0: the
-                                      general sort + chain-sum path.  Same results either way */
+    RDHIP_OPT_CHAIN = 16           /* member ordering of the evolve iterations: 1 (default): when
+                                      every segment (cloud, tile) has at most 2048 chosen points and
+                                      the segments have fewer chosen points than a scan block has
+                                      samples (on average), the scan counts the labels per block and
+                                      one scatter writes the members in chain order; 2: that path
+                                      whenever the 2048 limit allows; 0: the general stable sort.
+                                      Same results either way */
 };
 
 int rdhip_version(void);

@@ -641,7 +641,8 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
                 e.options.localMaxWork = value;
                 return RDHIP_OK;
             case RDHIP_OPT_CHAIN:
-                e.options.chain = value != 0;
+                if (value < 0 || value > 2) return fail(RDHIP_ERR_INVALID, "chain option must be 0, 1 or 2");
+                e.options.chain = (int)value;
                 return RDHIP_OK;
             case RDHIP_OPT_SUM_MODE:
                 if (value != 0 && value != 1) return fail(RDHIP_ERR_INVALID, "sum mode must be 0 (exact) or 1 (tolerance)");

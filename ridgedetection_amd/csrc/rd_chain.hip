@@ -329,13 +329,7 @@ hipError_t launch_chain_order(const ChainArgs<T>& a, int dim, int numItems, int 
     if (numItems <= 0 || a.totalRows <= 0) return hipSuccess;
     hipLaunchKernelGGL((key_prefix_kernel<T>), dim3((unsigned)((a.totalRows + 3) / 4)), dim3(256), 0, st, a);
     bool ok = false;
-    switch (dim) {
-        case 1: ok = launch_chain_dim<T, 1>(a, numItems, keyCapMax, st); break;
-        case 2: ok = launch_chain_dim<T, 2>(a, numItems, keyCapMax, st); break;
-        case 3: ok = launch_chain_dim<T, 3>(a, numItems, keyCapMax, st); break;
-        case 4: ok = launch_chain_dim<T, 4>(a, numItems, keyCapMax, st); break;
-        default: break;
-    }
+    RD_DISPATCH_DIM(dim, { ok = launch_chain_dim<T, DIM>(a, numItems, keyCapMax, st); });
     return ok ? hipGetLastError() : hipErrorInvalidValue;
 }
 

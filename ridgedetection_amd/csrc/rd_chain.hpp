@@ -11,7 +11,7 @@ namespace rdhip {
 // chosen points per segment the path handles (the scan's per-block label histogram lives in LDS)
 constexpr int kChainMaxKeys = 2048;
 // dimensions the path is built for (the scatter keeps DIM coordinates per sample in registers)
-constexpr int kChainMaxDim = 4;
+constexpr int kChainMaxDim = kMaxDim;
 
 // Chains are laid out in key order, chain k in [chainOff[k], chainEnd[k]) with every chain END
 // aligned to kChainAlign<T> values (16 bytes).  Gaps between chains are never read.

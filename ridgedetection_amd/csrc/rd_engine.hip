@@ -436,8 +436,11 @@ void Engine<T>::ensureChainScratch() {
     std::vector<int64_t> histOff(ns), memOff(ns + 1);
     int rows = 0, capMax = 0;
     int64_t h = 0, m = 0;
+    double keySamples = 0, samples = 0;
     for (int s = 0; s < ns; ++s) {
         cap[s] = std::max(K[s], 0);
+        keySamples += (double)cap[s] * segs_[s].nP;
+        samples += segs_[s].nP;
         capMax = std::max(capMax, cap[s]);
         rowOff[s] = rows;
         rows += cap[s];
@@ -474,6 +477,7 @@ void Engine<T>::ensureChainScratch() {
     chainMStride_ = m;
     chainTotalRows_ = rows;
     chainKeyCapMax_ = std::max(capMax, 1);
+    chainMeanKeys_ = samples > 0 ? keySamples / samples : 0;
     chainEpoch_ = countsEpoch_;
     chainReady_ = true;
 }
@@ -482,8 +486,12 @@ template <typename T>
 bool Engine<T>::chainUsable(int itemSize) const {
     if (!chainReady_ || chainEpoch_ != countsEpoch_ || !options.chain || options.tolerance || segmentedNow_)
         return false;
-    // items of itemSize samples from the starts of their segments (the count tables' columns)
-    return itemsPer_ == itemSize && chain_iteration_supported(dim_, itemSize, chainKeyCapMax_);
+    // items of itemSize samples from the starts of their segments (the count tables' columns);
+    // every block writes and the prefix pass rewrites a count per key of its segment, so the
+    // path pays only while a block holds more samples than its segment has keys (on average,
+    // weighted by samples): many keys per block make the general sort the cheaper ordering
+    return itemsPer_ == itemSize && chain_iteration_supported(dim_, itemSize, chainKeyCapMax_) &&
+           (options.chain == 2 || chainMeanKeys_ <= itemSize);
 }
 
 // rd_chain.hip's prefix and scatter after a histogram scan into labels.  Chains of at least
@@ -905,7 +913,7 @@ void Engine<T>::autotuneScan() {
     int64_t kTotal = 0;
     for (int k : K) kTotal += k;
     const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)),
-                                     ilog2(kTotal + 1), options.chain && !options.tolerance ? -1 : 0);
+                                     ilog2(kTotal + 1), options.chain != 0 && !options.tolerance ? -1 : 0);
     std::pair<int, int> best{options.ppt, options.scanThreads};
     bool cached = false;
     {

@@ -101,9 +101,10 @@ struct RunOptions {
                                     // reference's serial order); segment-local ones stay exact
     long long evolveGuard = 100000; // an evolve still moving after this many iterations fails
                                     // (0 = no limit, as in the reference)
-    bool chain = true;              // segments of at most kChainMaxKeys chosen points, dim <=
-                                    // kChainMaxDim: label histogram in the scan and one member
-                                    // scatter instead of the sort (rd_chain.hip)
+    int chain = 1;                  // member ordering without the sort (rd_chain.hip) for
+                                    // segments of at most kChainMaxKeys chosen points: 1 where it
+                                    // pays (fewer keys per segment than samples per block), 2
+                                    // always, 0 never
 };
 
 template <typename T>
@@ -261,6 +262,7 @@ private:
     DevBuf<int64_t> chainSegHistOff_, chainSegMemberOff_, chainTotalPos_;
     int64_t chainMStride_ = 0;
     int chainTotalRows_ = 0, chainKeyCapMax_ = 0;
+    double chainMeanKeys_ = 0;   // keys per segment, averaged over samples
     bool chainReady_ = false;
     long long chainEpoch_ = -1, countsEpoch_ = 0;   // countsEpoch_: bumped when chosen counts change
     int itemsPer_ = 0;                              // samples per work item of buildWorkItems

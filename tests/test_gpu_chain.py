@@ -22,7 +22,7 @@ def _say(msg):
 def _ctx(P, chain=True, shape=None):
     from ridgedetection_amd._lib import Context, OPT_CHAIN, OPT_PPT, OPT_SCAN_THREADS
     c = Context(P.shape[1], P.dtype)
-    c.set_option(OPT_CHAIN, 1 if chain else 0)
+    c.set_option(OPT_CHAIN, 2 if chain else 0)   # 2: the path whenever the key limit allows
     if shape:
         c.set_option(OPT_PPT, shape[0])
         c.set_option(OPT_SCAN_THREADS, shape[1])
@@ -152,13 +152,13 @@ def test_batches_take_the_path(oracle, dim):
     clouds = _batch_clouds(dim, 16, 500 + dim)
     _say(f"case chain batch d={dim} clouds={len(clouds)}")
     res = {}
-    for chain in (1, 0):
+    for chain in (2, 0):
         with Context(dim, np.float32) as c:
             c.set_option(OPT_CHAIN, chain)
             c.set_option(OPT_LOCAL, 0)
             res[chain] = c.batch([p for p, _, _ in clouds], [a for _, a, _ in clouds], [b for _, _, b in clouds])
-            assert c.timers()["chain_path"] == chain
-    for (P, r1, r2), S, G in zip(clouds, res[1], res[0]):
+            assert c.timers()["chain_path"] == (1 if chain else 0)
+    for (P, r1, r2), S, G in zip(clouds, res[2], res[0]):
         np.testing.assert_array_equal(S, G)
         np.testing.assert_array_equal(S, oracle.ridge_detection(P, r1, r2)[0])
 
@@ -171,12 +171,30 @@ def test_tiles_take_the_path(oracle, alg):
     P = synth.helix3d(30000, seed=33)
     _say(f"case chain tiled alg={alg}")
     res = {}
-    for chain in (1, 0):
+    for chain in (2, 0):
         with Context(3, np.float32) as c:
             c.set_option(OPT_CHAIN, chain)
             c.set_option(OPT_LOCAL, 0)
             c.set_points(P)
             res[chain] = c.tiled(alg, 0.2, 0.4, 3000, [4, 4, 4])
-            assert c.timers()["chain_path"] == chain
-    np.testing.assert_array_equal(res[1], res[0])
-    np.testing.assert_array_equal(res[1], oracle.tiled(alg, P, 0.2, 0.4, 3000, [4, 4, 4]))
+            assert c.timers()["chain_path"] == (1 if chain else 0)
+    np.testing.assert_array_equal(res[2], res[0])
+    np.testing.assert_array_equal(res[2], oracle.tiled(alg, P, 0.2, 0.4, 3000, [4, 4, 4]))
+
+
+def test_six_dimensional_tiles(oracle):
+    """A 6-D Bezier union tiled with the grouped tree (mixed RD) through the path, against the
+    general path and the oracle."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN, OPT_LOCAL
+    P = synth.bezier_union(20000, dim=6, seed=41)
+    _say("case chain tiled 6-D grouped")
+    res = {}
+    for chain in (2, 0):
+        with Context(6, np.float32) as c:
+            c.set_option(OPT_CHAIN, chain)
+            c.set_option(OPT_LOCAL, 0)
+            c.set_points(P)
+            res[chain] = c.tiled(1, 0.08, 0.16, 2000, [3] * 6)
+            assert c.timers()["chain_path"] == (1 if chain else 0)
+    np.testing.assert_array_equal(res[2], res[0])
+    np.testing.assert_array_equal(res[2], oracle.tiled(1, P, 0.08, 0.16, 2000, [3] * 6))
