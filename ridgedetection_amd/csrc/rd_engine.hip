@@ -726,27 +726,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
     // labels are live slots < keyBound_ (counts only shrink between refreshes), or in a
     // batch their indices in the live-slot list
     const int64_t kb = dense ? numRowSlots_ : std::max<int64_t>(keyBound_, 1);
-    // members sorted by (label, position); the last pass writes their coordinates in
-    // chain order straight into sortedP_
-    const SortGather<T> sg{es.P, es.stride, es.index, sortedP_.get(), totalP_, dim_};
-    check(stable_sort_by_key<T>(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr, dN_.get(),
-                                totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
-                                sortWs_.get(), st_, flags_.get(), sg),
-          "sort");
-    const int longShift = long_chain_shift(dim_, totalP);
-    const int sortKeys = (int)std::max<int64_t>(kb, 2);
-    if (sort_single_pass(sortKeys)) {
-        // every chosen slot's member run straight from the one-pass sort's scanned histogram
-        check(launch_key_ranges_from_sort(sortWs_.get(), totalP, sortKeys, dM_.get(), chainOff_.get(),
-                                          chainEnd_.get(), longList_.get(), longShift, flags_.get(), st_),
-              "key ranges");
-    } else {
-        hipLaunchKernelGGL(zero_pair_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), chainEnd_.get(),
-                           kb, flags_.get());
-        check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
-                                  longShift, flags_.get(), st_),
-              "chain_bounds");
-    }
+    const int longShift = enqueueSortOrder(labels, totalP, kb);
     ShiftArgs<T> ha{es.P, es.stride, es.index, S_.get(), sStride_, kb, dSlotSeg_.get(), dSegs_.get(),
                     dSegK_.get(), dSegDone_.get(), dSegMoved_.get(), members_.get(), dM_.get(), sortedP_.get(),
                     totalP_, chainOff_.get(), chainEnd_.get(), flags_.get(),
@@ -774,6 +754,33 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
                                      flags_.get(), flags_.get() + 1, st_),
               "finalize");
     if (ev) check(hipEventRecord(events_[3 * slot + 2], st_), "event");
+}
+
+template <typename T>
+int Engine<T>::enqueueSortOrder(const int* labels, int64_t totalP, int64_t kb) {
+    const EvolveSource es = evolveSource();
+    // members sorted by (label, position); the last pass writes their coordinates in
+    // chain order straight into sortedP_
+    const SortGather<T> sg{es.P, es.stride, es.index, sortedP_.get(), totalP_, dim_};
+    check(stable_sort_by_key<T>(compact() ? cKeys_.get() : labels, compact() ? cPos_.get() : nullptr, dN_.get(),
+                                totalP, (int)std::max<int64_t>(kb, 2), keys_.get(), members_.get(), dM_.get(),
+                                sortWs_.get(), st_, flags_.get(), sg),
+          "sort");
+    const int longShift = long_chain_shift(dim_, totalP);
+    const int sortKeys = (int)std::max<int64_t>(kb, 2);
+    if (sort_single_pass(sortKeys)) {
+        // every chosen slot's member run straight from the one-pass sort's scanned histogram
+        check(launch_key_ranges_from_sort(sortWs_.get(), totalP, sortKeys, dM_.get(), chainOff_.get(),
+                                          chainEnd_.get(), longList_.get(), longShift, flags_.get(), st_),
+              "key ranges");
+    } else {
+        hipLaunchKernelGGL(zero_pair_kernel, dim3(grid_for(kb)), dim3(256), 0, st_, chainOff_.get(), chainEnd_.get(),
+                           kb, flags_.get());
+        check(launch_chain_bounds(keys_.get(), dM_.get(), totalP, chainOff_.get(), chainEnd_.get(), longList_.get(),
+                                  longShift, flags_.get(), st_),
+              "chain_bounds");
+    }
+    return longShift;
 }
 
 template <typename T>
@@ -944,9 +951,10 @@ void Engine<T>::autotuneScan() {
                 options.scanThreads = cands[c][1];
                 buildWorkItems();
                 sa.items = dItems_.get();
-                // the form the iterations will run: with the label histogram and the member
-                // scatter (whose cost depends on the item size) when they take the rd_chain.hip
-                // path; the ordered sums after it do not depend on the shape
+                // the scan and the member ordering the iterations will run with this shape: the
+                // label histogram and rd_chain.hip's scatter (whose cost depends on the item
+                // size) where that path is usable, else the sort; the ordered sums after either
+                // cost about the same
                 const int itemSize = cands[c][0] * cands[c][1];
                 const bool hist = chainUsable(itemSize);
                 sa.blockHist = hist ? chainHist_.get() : nullptr;
@@ -955,7 +963,10 @@ void Engine<T>::autotuneScan() {
                 sa.histItem = hist ? itemSize : 0;
                 auto pass = [&] {
                     check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
-                    if (hist) enqueueChainOrder(itemSize, labels_.get());
+                    if (hist)
+                        enqueueChainOrder(itemSize, labels_.get());
+                    else
+                        (void)enqueueSortOrder(labels_.get(), totalP_, std::max<int64_t>(keyBound_, 1));
                 };
                 pass();
                 check(hipEventRecord(e0, st_), "event");

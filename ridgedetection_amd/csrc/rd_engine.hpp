@@ -309,6 +309,10 @@ private:
     void ensureChainScratch();
     void enqueueChainOrder(int itemSize, const int* labels);
     bool chainUsable(int itemSize) const;
+    // the general member ordering: stable sort of the scan's keys over totalP positions (keys
+    // below kb), member coordinates into sortedP_, chain ranges and the wave list; returns the
+    // wave list's shift
+    int enqueueSortOrder(const int* labels, int64_t totalP, int64_t kb);
     // finalize: also the per-segment convergence bookkeeping (iteration cap capIter)
     void autotuneBatchScan();
     void enqueueIteration(bool record, int slot = -1, const IterLog* log = nullptr, bool finalize = false,
