@@ -187,6 +187,7 @@ def batched_vs_looped(count=512, loop_every=1, seed=7, graph=True, batch_only=Fa
     tb = ctxs[dmax].timers()
     shape = {"samples_per_lane": tb["scan_ppt"], "threads_per_block": tb["scan_threads"],
              "persistent": bool(tb["scan_persistent"]), "tune_ms": tb["tune_ms"],
+             "screen": SCREEN_FORMS[tb["screen_form"]],
              "member_ordering": "per-segment scatter" if tb["chain_path"] else "sort"}
     log(f"bench: batched {count} clouds in {t_batch:.2f} s")
     if batch_only:
@@ -332,6 +333,30 @@ def general_path(P, r, steps):
     return {"ms_per_step": el * 1e3 / steps, "pair_evals_per_s": None}
 
 
+SCREEN_FORMS = {0: "direct", 1: "expanded", 2: "expanded on the matrix cores"}
+
+
+def screen_forms(P, r, steps):
+    """The same iterations with each evolve-scan screen forced (RDHIP_OPT_SCREEN 0 direct, 1
+    expanded, 3 expanded on the matrix cores; the scan shape autotuned within the form): ms per
+    iteration, graph replay."""
+    from ridgedetection_amd._lib import Context, OPT_SCREEN
+    out = {}
+    for opt, form in ((0, 0), (1, 1), (3, 2)):
+        with Context(P.shape[1], P.dtype) as c:
+            c.set_option(OPT_SCREEN, opt)
+            c.set_points(P)
+            c.choose(r)
+            c.evolve_fixed(r, 8)
+            t0 = time.perf_counter()
+            c.evolve_fixed(r, steps)
+            el = time.perf_counter() - t0
+            t = c.timers()
+        out[SCREEN_FORMS[form]] = {"ms_per_step": el * 1e3 / steps,
+                                   "scan_shape": {"samples_per_lane": t["scan_ppt"], "threads_per_block": t["scan_threads"]}}
+    return out
+
+
 def fp64_helix(n=1_000_000, steps=40, warmup=8, r=0.2):
     """The headline workload in FP64 (same seeded helix, cast): scan kernel and whole
     iteration times, against the FP64 vector peak."""
@@ -359,7 +384,8 @@ def fp64_helix(n=1_000_000, steps=40, warmup=8, r=0.2):
             "scan_kernel_ms": scan_ms, "evolve_iteration_device_ms": iter_ms,
             "scan_TFLOPs": flops / (scan_ms * 1e-3) / 1e12,
             "scan_frac_of_fp64_peak": flops / (scan_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-            "scan_shape": {"samples_per_lane": ta["scan_ppt"], "threads_per_block": ta["scan_threads"]}}
+            "scan_shape": {"samples_per_lane": ta["scan_ppt"], "threads_per_block": ta["scan_threads"],
+                           "screen": SCREEN_FORMS[ta["screen_form"]]}}
 
 
 def fp32_vs_fp64():
@@ -507,7 +533,7 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
                 "evolve_pair_evals_per_s": pps, "evolve_TFLOPs": tflops,
                 "evolve_frac_of_fp32_valu_peak": tflops / FP32_PEAK_TFLOPS,
                 "scan_shape": {"samples_per_lane": t["scan_ppt"], "threads_per_block": t["scan_threads"],
-                               "persistent": bool(t["scan_persistent"])}}
+                               "persistent": bool(t["scan_persistent"]), "screen": SCREEN_FORMS[t["screen_form"]]}}
     return out
 
 
@@ -678,8 +704,9 @@ def main():
         "config": {"workload": "helix 1M 3-D points, sigma 0.05, r 0.2, brute-force evolve iteration",
                    "points": n, "dim": d, "r": r, "chosen": K,
                    "scan_shape": {"samples_per_lane": t_after["scan_ppt"], "threads_per_block": t_after["scan_threads"],
-                                  "autotuned": not args.ppt}},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                                  "screen": SCREEN_FORMS[t_after["screen_form"]], "autotuned": not args.ppt}},
+        "roofline": {"bound": "mfma" if t_after["screen_form"] == 2 else "valu", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(n, args.dtype),
                      "kernel": "evolve_scan_hist_kernel" if t_after["chain_path"] else "evolve_scan_kernel",
                      "avg_launch_ms": scan_ms,
@@ -694,6 +721,8 @@ def main():
     gp = general_path(P, r, args.steps)
     gp["pair_evals_per_s"] = pairs * 1e3 / gp["ms_per_step"]
     result["secondary"]["general_ordering_path"] = gp
+    log("bench: the same iterations with each screen form forced")
+    result["secondary"]["screen_forms"] = screen_forms(P, r, args.steps)
     if dt == np.float32:
         log("bench: FP64 helix and FP32-vs-FP64 errors")
         result["secondary"]["fp64_helix_1M"] = fp64_helix(n=args.n)

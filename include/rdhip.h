@@ -101,6 +101,8 @@ typedef struct rdhip_timers {
     int64_t vjp_checkpoint_every;  /* iterations per recorded run of the last rdhip_evolve_fwd */
     int64_t chain_path;      /* 1: the last evolve iteration enqueued ordered its members with the
                                 single-cloud path (RDHIP_OPT_CHAIN), 0: with the general sort */
+    int64_t screen_form;     /* screen of the last evolve scan enqueued (RDHIP_OPT_SCREEN):
+                                0 direct, 1 expanded, 2 expanded on the matrix cores */
 } rdhip_timers;
 
 /* Tuning / instrumentation options (rdhip_set_option). */
@@ -156,13 +158,22 @@ enum rdhip_option {
                                       the mean of that sum -- no serial floor on long chains,
                                       deterministic, but means differ from the reference's by its
                                       own rounding of the serial sum */
-    RDHIP_OPT_CHAIN = 16           /* member ordering of the evolve iterations: 1 (default): when
+    RDHIP_OPT_CHAIN = 16,          /* member ordering of the evolve iterations: 1 (default): when
                                       every segment (cloud, tile) has at most 2048 chosen points and
                                       the segments have fewer chosen points than a scan block has
                                       samples (on average), the scan counts the labels per block and
                                       one scatter writes the members in chain order; 2: that path
                                       whenever the 2048 limit allows; 0: the general stable sort.
                                       Same results either way */
+    RDHIP_OPT_SCREEN = 17          /* how the evolve scan screens (sample, chosen point) pairs
+                                      before its exact decision: 2 (default): the fastest form as
+                                      timed by the autotuner (0 where it does not run); 1:
+                                      expanded (|s-o|^2 - 2 (s-o).(p-o) about a
+                                      per-segment centre o, DIM fused multiply-adds per pair, a
+                                      rigorous bound and an exact pass for near ties); 3: the
+                                      expanded form on the matrix cores (FP32; FP64 uses 1); 0:
+                                      direct (|s-p|^2 with a relative bound).  Same results
+                                      whichever form runs */
 };
 
 int rdhip_version(void);

@@ -40,6 +40,7 @@ OPT_PPT, OPT_ITERS_PER_SYNC, OPT_PROFILE, OPT_PERSISTENT, OPT_GRAPH, OPT_LOCAL =
 OPT_LOCAL_MAX_SAMPLES, OPT_LOCAL_MAX_WORK, OPT_SCAN_THREADS, OPT_AUTOTUNE, OPT_SEGMENTED = 6, 7, 8, 9, 10
 OPT_POSITION_COPY, OPT_EVOLVE_GUARD, OPT_INJECT_ALLOC_FAILURE, OPT_VJP_MEMORY_MB, OPT_SUM_MODE = 11, 12, 13, 14, 15
 OPT_CHAIN = 16
+OPT_SCREEN = 17
 GEN_CIRCLE, GEN_SPHERE, GEN_SPIRAL2D, GEN_SPIRAL3D, GEN_SEGMENT, GEN_HELIX, GEN_POLYLINE, GEN_UNIFORM = range(8)
 
 
@@ -51,7 +52,8 @@ class Timers(C.Structure):
                 ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("evolve_bytes", C.c_double),
                 ("scan_ppt", C.c_int64), ("scan_threads", C.c_int64), ("tune_ms", C.c_double),
                 ("evolve_pairs", C.c_double), ("scan_persistent", C.c_int64),
-                ("vjp_checkpoint_every", C.c_int64), ("chain_path", C.c_int64)]
+                ("vjp_checkpoint_every", C.c_int64), ("chain_path", C.c_int64),
+                ("screen_form", C.c_int64)]
 
 
 _lib = None

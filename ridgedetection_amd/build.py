@@ -28,9 +28,14 @@ FLAGS = [
 ]
 
 
+# per-file flags: the matrix-core scan keeps its accumulators in VGPRs (the VALU reads them for
+# the running minima; the AGPR form would copy every one out first)
+FILE_FLAGS = {"rd_scan_m_f32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -509,6 +509,7 @@ int rdhip_get_timers(rdhip_ctx* ctx, rdhip_timers* out) {
         out->scan_persistent = e.timers.scanPersistent;
         out->vjp_checkpoint_every = e.timers.vjpCheckpointEvery;
         out->chain_path = e.timers.chainPath;
+        out->screen_form = e.timers.screenForm;
         return RDHIP_OK;
     });
 }
@@ -643,6 +644,10 @@ int rdhip_set_option(rdhip_ctx* ctx, int option, int64_t value) {
             case RDHIP_OPT_CHAIN:
                 if (value < 0 || value > 2) return fail(RDHIP_ERR_INVALID, "chain option must be 0, 1 or 2");
                 e.options.chain = (int)value;
+                return RDHIP_OK;
+            case RDHIP_OPT_SCREEN:
+                if (value < 0 || value > 3) return fail(RDHIP_ERR_INVALID, "screen option must be 0, 1, 2 or 3");
+                e.options.screen = (int)value;
                 return RDHIP_OK;
             case RDHIP_OPT_SUM_MODE:
                 if (value != 0 && value != 1) return fail(RDHIP_ERR_INVALID, "sum mode must be 0 (exact) or 1 (tolerance)");

@@ -48,6 +48,8 @@ template <> struct FpTraits<float> {
     static constexpr float sentinel = 1.0e18f;                 // (1e18)^2 * 12 < FLT_MAX
     static constexpr float big = 3.402823466e+38f;             // numeric_limits<float>::max()
     static constexpr float tiny = 1.0e-35f;
+    static constexpr float xSentinel = 1.0e30f;                // expanded-screen value of empty slots
+    static constexpr float minNormal = 1.17549435e-38f;        // numeric_limits<float>::min()
 };
 template <> struct FpTraits<double> {
     static constexpr double eps = 2.220446049250313080847e-16;  // 2^-52
@@ -55,6 +57,8 @@ template <> struct FpTraits<double> {
     static constexpr double sentinel = 1.0e150;
     static constexpr double big = 1.7976931348623157e+308;
     static constexpr double tiny = 1.0e-300;
+    static constexpr double xSentinel = 1.0e300;
+    static constexpr double minNormal = 2.2250738585072014e-308;
 };
 
 // Per-segment static description (host-built, uploaded once per run).

@@ -415,6 +415,8 @@ void Engine<T>::ensureEvolveScratch() {
     sums_.reserve((size_t)sStride_ * dim_);
     sortWs_.reserve(sort_workspace_bytes(totalP, (int)std::max<int64_t>(totalSlots_, 2)));
     sortedP_.reserve((size_t)totalP * dim_);
+    xTable_.reserve((size_t)(dim_ + 1) * std::max<int64_t>(sStride_, 1));
+    segX_.reserve((size_t)std::max<size_t>(segs_.size(), 1) * kScreenStride);
     check(hipMemcpyAsync(dN_.get(), &totalP, sizeof(int64_t), hipMemcpyHostToDevice, st_), "n");
     // decimate scratch: elemBase, groupOff/Cnt/Left, elemSlot, elemGroup, c1, c2, rowOff, alive
     const int ns = std::max<int>((int)segs_.size(), 1);
@@ -561,6 +563,7 @@ void Engine<T>::compactSlots() {
     check(hipMemcpyAsync(dSlotSeg_.get(), slotSeg.data(), sizeof(int) * totalSlots_, hipMemcpyHostToDevice, st_),
           "slots");
     sums_.reserve((size_t)sStride_ * dim_);
+    xTable_.reserve((size_t)(dim_ + 1) * sStride_);
     check(hipStreamSynchronize(st_), "relayout");
     updateKeyBound();
 }
@@ -656,6 +659,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
         ScanArgs<T> sa{es.P, es.stride, es.index, S_.get(), sStride_, dItems_.get(), dSegs_.get(),
                        dSegK_.get(), dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
                        nullptr, nullptr, nullptr};
+        prepScreen(sa, screenForm());
         check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
         if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
         LocalArgs<T> la{es.P, es.stride, es.index, S_.get(), sStride_, dSegs_.get(), dSegK_.get(),
@@ -679,6 +683,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
                        dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
                        nullptr, nullptr, compact() ? flags_.get() + 4 : nullptr, nullptr, chainHist_.get(),
                        chainSegHistOff_.get(), chainSegKeyCap_.get(), itemSize};
+        prepScreen(sa, screenForm());
         check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, 0, st_), "evolve_scan");
         if (ev) check(hipEventRecord(events_[3 * slot + 1], st_), "event");
         enqueueChainOrder(itemSize, labels);
@@ -714,6 +719,7 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
                    dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels, flags_.get() + 3,
                    compact() ? cKeys_.get() : nullptr, compact() ? cPos_.get() : nullptr,
                    compact() ? flags_.get() + 4 : nullptr, dense ? denseOf_.get() : nullptr};
+    prepScreen(sa, screenForm());
     // one block per work item (batches: the active-item list, or persistent blocks on its
     // queue when that form is selected)
     check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads,
@@ -757,6 +763,20 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
 }
 
 template <typename T>
+void Engine<T>::prepScreen(ScanArgs<T>& sa, int form) {
+    sa.xTable = nullptr;
+    sa.segX = nullptr;
+    sa.mfma = form == 2 ? 1 : 0;
+    timers.screenForm = form;
+    if (form == 0) return;
+    check(launch_expand_chosen<T>(S_.get(), sStride_, dSegs_.get(), dSegK_.get(), dSegDone_.get(), (int)segs_.size(),
+                                  dim_, xTable_.get(), segX_.get(), flags_.get(), st_),
+          "expand_chosen");
+    sa.xTable = xTable_.get();
+    sa.segX = segX_.get();
+}
+
+template <typename T>
 int Engine<T>::enqueueSortOrder(const int* labels, int64_t totalP, int64_t kb) {
     const EvolveSource es = evolveSource();
     // members sorted by (label, position); the last pass writes their coordinates in
@@ -797,13 +817,19 @@ void Engine<T>::readEvents(int used) {
 
 namespace {
 std::mutex g_tuneMutex;
-std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> g_tuned;   // -> (ppt, threads)
+struct ScanShape {
+    int ppt;
+    int threads;
+    int form;   // screen form (Engine::screenForm)
+};
+std::map<std::tuple<int, int, int, int, int, int>, ScanShape> g_tuned;
 struct BatchShape {
     int ppt;
     int threads;
     bool persistent;
+    int form;
 };
-std::map<std::tuple<int, int, int, int, int>, BatchShape> g_tunedBatch;
+std::map<std::tuple<int, int, int, int, int, int>, BatchShape> g_tunedBatch;
 inline int ilog2(int64_t v) {
     int l = 0;
     while (v > 1) {
@@ -826,8 +852,10 @@ void Engine<T>::autotuneBatchScan() {
     int64_t kTotal = 0;
     for (int k : K) kTotal += k;
     const int ns = (int)segs_.size();
-    const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)), 0, 1 + ilog2(ns));
-    BatchShape best{options.ppt, options.scanThreads, options.persistent};
+    const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)), 0, 1 + ilog2(ns),
+                                     options.screen);
+    const std::vector<int> forms = screenForms();
+    BatchShape best{options.ppt, options.scanThreads, options.persistent, forms[0]};
     bool cached = false;
     {
         std::lock_guard<std::mutex> g(g_tuneMutex);
@@ -838,22 +866,38 @@ void Engine<T>::autotuneBatchScan() {
         }
     }
     if (!cached) {
-        static const BatchShape cands[] = {{4, 256, true},  {2, 256, true},  {1, 256, true},
-                                           {4, 256, false}, {2, 256, false}, {1, 256, false},
-                                           {2, 512, false}, {1, 512, false}, {4, 128, false}};
-        constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
+        // shapes in the first screen form allowed (the direct one where the tuner picks the
+        // form), then the best shape in each other form allowed
+        static const BatchShape shapes[] = {{4, 256, true, 0},  {2, 256, true, 0},  {1, 256, true, 0},
+                                            {4, 256, false, 0}, {2, 256, false, 0}, {1, 256, false, 0},
+                                            {2, 512, false, 0}, {1, 512, false, 0}, {4, 128, false, 0}};
+        constexpr int kShapes = (int)(sizeof(shapes) / sizeof(shapes[0]));
+        BatchShape cands[kShapes + 2];
+        for (int c = 0; c < kShapes; ++c) {
+            cands[c] = shapes[c];
+            cands[c].form = forms[0];
+        }
+        const int kCands = kShapes + (int)forms.size() - 1;
         const EvolveSource es = evolveSource();
         const std::vector<int> none(ns, 0);
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
-        float score[kCands];
+        float score[kShapes + 2];
         for (int c = 0; c < kCands; ++c) score[c] = 1e30f;
         // two rounds of one untimed and one timed launch per shape (one round when a launch
         // takes more than 4 ms: the tuning then costs about 20 scans of the batch)
         int rounds = 2;
         for (int round = 0; round < rounds; ++round) {
             for (int c = 0; c < kCands; ++c) {
+                if (c >= kShapes && round == 0) {
+                    // another form at the best shape of the first round
+                    int b = 0;
+                    for (int i = 1; i < kShapes; ++i)
+                        if (score[i] < score[b]) b = i;
+                    cands[c] = cands[b];
+                    cands[c].form = forms[c - kShapes + 1];
+                }
                 options.ppt = cands[c].ppt;
                 options.scanThreads = cands[c].threads;
                 options.persistent = cands[c].persistent;
@@ -864,8 +908,10 @@ void Engine<T>::autotuneBatchScan() {
                                dSegDone_.get(), dSegR1Sqr_.get(), flags_.get(), labels_.get(), flags_.get() + 3,
                                cKeys_.get(), cPos_.get(), flags_.get() + 4, nullptr};
                 const int pb = persistentBlocks();
+                prepScreen(sa, cands[c].form);
                 check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, pb, st_), "tune");
                 check(hipEventRecord(e0, st_), "event");
+                prepScreen(sa, cands[c].form);
                 check(launch_evolve_scan<T>(sa, dim_, numItems_, options.ppt, options.scanThreads, pb, st_), "tune");
                 check(hipEventRecord(e1, st_), "event");
                 check(hipEventSynchronize(e1), "event");
@@ -895,6 +941,7 @@ void Engine<T>::autotuneBatchScan() {
         orderItemsByCost();
         check(hipStreamSynchronize(st_), "autotune");
     }
+    if (options.screen == 2) tunedForm_ = best.form;
     timers.scanPpt = options.ppt;
     timers.scanThreads = options.scanThreads;
     timers.scanPersistent = options.persistent ? 1 : 0;
@@ -920,8 +967,10 @@ void Engine<T>::autotuneScan() {
     int64_t kTotal = 0;
     for (int k : K) kTotal += k;
     const auto key = std::make_tuple((int)sizeof(T), dim_, ilog2(std::max<int64_t>(totalP_, 1)),
-                                     ilog2(kTotal + 1), options.chain != 0 && !options.tolerance ? -1 : 0);
-    std::pair<int, int> best{options.ppt, options.scanThreads};
+                                     ilog2(kTotal + 1), options.chain != 0 && !options.tolerance ? -1 : 0,
+                                     options.screen);
+    const std::vector<int> forms = screenForms();
+    ScanShape best{options.ppt, options.scanThreads, forms[0]};
     bool cached = false;
     {
         std::lock_guard<std::mutex> g(g_tuneMutex);
@@ -932,8 +981,14 @@ void Engine<T>::autotuneScan() {
         }
     }
     if (!cached) {
-        static const int cands[][2] = {{4, 256}, {2, 256}, {1, 256}, {4, 128}, {2, 128}, {1, 128},
-                                       {4, 512}, {2, 512}, {1, 512}};
+        static const int shapes[][2] = {{4, 256}, {2, 256}, {1, 256}, {4, 128}, {2, 128}, {1, 128},
+                                        {4, 512}, {2, 512}, {1, 512}};
+        constexpr int kShapes = (int)(sizeof(shapes) / sizeof(shapes[0]));
+        // every shape in every screen form allowed
+        ScanShape cands[3 * kShapes];
+        int kCands = 0;
+        for (int f : forms)
+            for (int c = 0; c < kShapes; ++c) cands[kCands++] = {shapes[c][0], shapes[c][1], f};
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
@@ -942,27 +997,27 @@ void Engine<T>::autotuneScan() {
                        nullptr, nullptr, nullptr};
         // three rounds over all candidates (so that no shape always runs first), each
         // launch pair timed after one untimed launch; a shape scores its best round
-        constexpr int kCands = (int)(sizeof(cands) / sizeof(cands[0]));
-        float score[kCands];
+        float score[3 * kShapes];
         for (int c = 0; c < kCands; ++c) score[c] = 1e30f;
         for (int round = 0; round < 3; ++round) {
             for (int c = 0; c < kCands; ++c) {
-                options.ppt = cands[c][0];
-                options.scanThreads = cands[c][1];
+                options.ppt = cands[c].ppt;
+                options.scanThreads = cands[c].threads;
                 buildWorkItems();
                 sa.items = dItems_.get();
                 // the scan and the member ordering the iterations will run with this shape: the
                 // label histogram and rd_chain.hip's scatter (whose cost depends on the item
                 // size) where that path is usable, else the sort; the ordered sums after either
                 // cost about the same
-                const int itemSize = cands[c][0] * cands[c][1];
+                const int itemSize = cands[c].ppt * cands[c].threads;
                 const bool hist = chainUsable(itemSize);
                 sa.blockHist = hist ? chainHist_.get() : nullptr;
                 sa.histSegOff = hist ? chainSegHistOff_.get() : nullptr;
                 sa.histKeyCap = hist ? chainSegKeyCap_.get() : nullptr;
                 sa.histItem = hist ? itemSize : 0;
                 auto pass = [&] {
-                    check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c][0], cands[c][1], 0, st_), "tune");
+                    prepScreen(sa, cands[c].form);
+                    check(launch_evolve_scan<T>(sa, dim_, numItems_, cands[c].ppt, cands[c].threads, 0, st_), "tune");
                     if (hist)
                         enqueueChainOrder(itemSize, labels_.get());
                     else
@@ -982,20 +1037,21 @@ void Engine<T>::autotuneScan() {
         for (int c = 0; c < kCands; ++c)
             if (score[c] < bestMs) {
                 bestMs = score[c];
-                best = {cands[c][0], cands[c][1]};
+                best = cands[c];
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         std::lock_guard<std::mutex> g(g_tuneMutex);
         g_tuned[key] = best;
     }
-    if (options.ppt != best.first || options.scanThreads != best.second || !cached) {
-        options.ppt = best.first;
-        options.scanThreads = best.second;
+    if (options.ppt != best.ppt || options.scanThreads != best.threads || !cached) {
+        options.ppt = best.ppt;
+        options.scanThreads = best.threads;
         buildWorkItems();
         if (segs_.size() > 1) orderItemsByCost();
         check(hipStreamSynchronize(st_), "autotune");
     }
+    if (options.screen == 2) tunedForm_ = best.form;
     timers.scanPpt = options.ppt;
     timers.scanThreads = options.scanThreads;
     timers.tuneMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -76,6 +76,8 @@ struct Timers {
     int scanPersistent = 0;             // ... and whether its blocks pull items from a queue
     long long vjpCheckpointEvery = 0;   // iterations per recorded run of the last VJP forward
     int chainPath = 0;                  // the last iteration enqueued took the rd_chain.hip path
+    int screenForm = 0;                 // screen of the last scan enqueued: 0 direct, 1 expanded,
+                                        // 2 expanded on the matrix cores
     double tuneMs = 0;                  // time spent timing candidate shapes
 };
 
@@ -105,6 +107,10 @@ struct RunOptions {
                                     // segments of at most kChainMaxKeys chosen points: 1 where it
                                     // pays (fewer keys per segment than samples per block), 2
                                     // always, 0 never
+    int screen = 2;                 // evolve-scan screen (rd_nearest.hpp): 2 the fastest form as
+                                    // timed by the autotuner (direct where it does not run),
+                                    // 1 expanded, 3 expanded on the matrix cores (FP32; FP64:
+                                    // expanded), 0 direct
 };
 
 template <typename T>
@@ -266,6 +272,30 @@ private:
     bool chainReady_ = false;
     long long chainEpoch_ = -1, countsEpoch_ = 0;   // countsEpoch_: bumped when chosen counts change
     int itemsPer_ = 0;                              // samples per work item of buildWorkItems
+    // expanded-form screen: table rows (dim + 1) x sStride_ and per-segment centre and bound
+    // (rd_screen.hip), rebuilt before every scan that uses them; tunedForm_: the form the
+    // autotuner picked
+    DevBuf<T> xTable_, segX_;
+    static constexpr int kMatrixForm = sizeof(T) == 4 ? 2 : 1;   // the matrix-core form is FP32
+    int tunedForm_ = 0;
+    // screen form of the next scans (0 direct, 1 expanded, 2 expanded on the matrix cores)
+    int screenForm() const {
+        switch (options.screen) {
+            case 0: return 0;
+            case 1: return 1;
+            case 3: return kMatrixForm;
+            default: return tunedForm_;
+        }
+    }
+    // the forms the tuner may pick from under the current option (the batch tuner times its
+    // shapes in the first one, then the best shape in the others)
+    std::vector<int> screenForms() const {
+        if (options.screen != 2) return {screenForm()};
+        if (kMatrixForm == 2) return {0, 2, 1};
+        return {0, 1};
+    }
+    // enqueues the table for the scan when form > 0, and points sa at it (else clears those fields)
+    void prepScreen(ScanArgs<T>& sa, int form);
     int64_t keyBound_ = 0;   // 1 + highest live chosen slot (host copy, refreshed per phase)
     void updateKeyBound();
     DevBuf<unsigned char> moved_;   // [dim][totalSlots]

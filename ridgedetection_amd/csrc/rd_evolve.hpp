@@ -41,7 +41,24 @@ struct ScanArgs {
     const int64_t* histSegOff;
     const int* histKeyCap;
     int histItem;
+    // optional: the expanded-form screen (rd_nearest.hpp) over the table of expand_chosen
+    // (rows d < dim: s - o, row dim: |s - o|^2, stride sStride) and the per-segment
+    // centre and bound segX[seg * kScreenStride + d], segX[seg * kScreenStride + kMaxDim]
+    const T* xTable;
+    const T* segX;
+    int mfma;   // 1: the expanded screen on the matrix cores (FP32; nearest_labels_mfma)
 };
+
+constexpr int kScreenStride = kMaxDim + 1;
+
+// Per-segment centre o (midpoint of the live chosen points' bounding box), the expanded table
+// rows of every slot below the tile-rounded live count (slots past it: s' = 0, b = xSentinel)
+// and the largest b over the live slots; one block per segment, skipped for converged
+// segments and after the all-done flag is set.
+template <typename T>
+hipError_t launch_expand_chosen(const T* S, int64_t sStride, const SegDesc* segs, const int* segK,
+                                const int* segDone, int numSegs, int dim, T* xTable, T* segX, const int* allDone,
+                                hipStream_t st);
 
 template <typename T>
 struct ShiftArgs {
