@@ -890,8 +890,9 @@ void Engine<T>::autotuneBatchScan() {
         int rounds = 2;
         for (int round = 0; round < rounds; ++round) {
             for (int c = 0; c < kCands; ++c) {
-                if (c >= kShapes && round == 0) {
-                    // another form at the best shape of the first round
+                if (c >= kShapes) {
+                    // another form at the best shape of the first round (timed in that round only)
+                    if (round > 0) continue;
                     int b = 0;
                     for (int i = 1; i < kShapes; ++i)
                         if (score[i] < score[b]) b = i;

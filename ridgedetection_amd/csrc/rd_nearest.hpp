@@ -52,58 +52,8 @@ __device__ __forceinline__ T x_value(const T (&q2)[DIM], const T* s /* s[d] */, 
 // ---------------------------------------------------------------------------------
 // Main loops: per sample the best tile-minimum over tiles of kChosenTile chosen points, the
 // tile holding it (wt) and the second-best tile-minimum.  w holds the samples (direct screen)
-// or -2(p - o) (expanded screen, X); Tb is the chosen rows (direct) or the expanded table.
-//
-// The chosen values stream through SGPRs with uniform scalar loads.  Scalar loads return out
-// of order, so a wave can only wait for all of its outstanding ones at once.  DB = false loads
-// a whole tile and waits; DB = true splits a tile into sub-tiles of H chosen points in two
-// SGPR buffers and issues the next sub-tile's loads right after the current one's wait, so
-// they arrive while the current sub-tile is screened (H from the SGPR budget: 2 H rows <= 40).
-template <int DIM, bool X>
-constexpr int db_sub_tile() {
-    constexpr int rows = X ? DIM + 1 : DIM;
-    return rows <= 2 ? 8 : (rows <= 5 ? 4 : 2);
-}
-
-template <typename T, int DIM, int H, bool X>
-struct SubTile {
-    T s[H][DIM];
-    T b[H];
-};
-
-template <typename T, int DIM, int H, bool X>
-__device__ __forceinline__ void load_sub_tile(SubTile<T, DIM, H, X>& st, const T* __restrict__ Tb, int64_t ss,
-                                              int slot0) {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-#pragma unroll
-        for (int t = 0; t < H; ++t) st.s[t][d] = Tb[d * ss + slot0 + t];
-    if constexpr (X) {
-#pragma unroll
-        for (int t = 0; t < H; ++t) st.b[t] = Tb[DIM * ss + slot0 + t];
-    }
-}
-
-template <typename T, int DIM, int PPT, int H, bool X>
-__device__ __forceinline__ void sub_tile_pairs(const SubTile<T, DIM, H, X>& st, const T (&w)[PPT][DIM], int t0,
-                                               int t1, T (&m)[PPT]) {
-#pragma unroll
-    for (int t = t0; t < t1; t += 2) {
-#pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-            T a0, a1;
-            if constexpr (X) {
-                a0 = x_value<T, DIM>(w[j], st.s[t], st.b[t]);
-                a1 = x_value<T, DIM>(w[j], st.s[t + 1], st.b[t + 1]);
-            } else {
-                a0 = screen_dist<T, DIM>(w[j], st.s[t]);
-                a1 = screen_dist<T, DIM>(w[j], st.s[t + 1]);
-            }
-            m[j] = tile_min<T>(m[j], a0, a1);
-        }
-    }
-}
-
+// or -2(p - o) (expanded screen, X); Tb is the chosen rows (direct) or the expanded table.  The
+// chosen values stream through SGPRs with uniform scalar loads, a whole tile per wait.
 template <typename T, int PPT>
 __device__ __forceinline__ void tile_done(int tj, const T (&m)[PPT], T (&best)[PPT], T (&second)[PPT], int (&wt)[PPT]) {
 #pragma unroll
@@ -115,7 +65,7 @@ __device__ __forceinline__ void tile_done(int tj, const T (&m)[PPT], T (&best)[P
     }
 }
 
-template <typename T, int DIM, int PPT, bool X, bool DB>
+template <typename T, int DIM, int PPT, bool X>
 __device__ __forceinline__ void screen_main(const T (&w)[PPT][DIM], int K, const T* __restrict__ Tb, int64_t ss,
                                             T (&best)[PPT], T (&second)[PPT], int (&wt)[PPT]) {
     constexpr T big = FpTraits<T>::big;
@@ -126,35 +76,7 @@ __device__ __forceinline__ void screen_main(const T (&w)[PPT][DIM], int K, const
         wt[j] = 0;
     }
     const int nT = (K + kChosenTile - 1) / kChosenTile;
-    if constexpr (DB) {
-        constexpr int H = db_sub_tile<DIM, X>();
-        constexpr int NS = kChosenTile / H;   // even
-        SubTile<T, DIM, H, X> A, B;
-        load_sub_tile<T, DIM, H, X>(A, Tb, ss, 0);
-        for (int tj = 0; tj < nT; ++tj) {
-            T m[PPT];
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) m[j] = big;
-            // the last tile prefetches its own first sub-tile again (stays in bounds)
-            const int next = (tj + 1 < nT ? tj + 1 : tj) * kChosenTile;
-#pragma unroll
-            for (int g = 0; g < NS; g += 2) {
-                // sub-tile g in A: its first pair waits for A alone, then B's loads go out
-                sub_tile_pairs<T, DIM, PPT, H, X>(A, w, 0, 2, m);
-                __builtin_amdgcn_sched_barrier(0);
-                load_sub_tile<T, DIM, H, X>(B, Tb, ss, tj * kChosenTile + (g + 1) * H);
-                __builtin_amdgcn_sched_barrier(0);
-                sub_tile_pairs<T, DIM, PPT, H, X>(A, w, 2, H, m);
-                // sub-tile g + 1 in B; A takes sub-tile g + 2, or the next tile's first
-                sub_tile_pairs<T, DIM, PPT, H, X>(B, w, 0, 2, m);
-                __builtin_amdgcn_sched_barrier(0);
-                load_sub_tile<T, DIM, H, X>(A, Tb, ss, g + 2 < NS ? tj * kChosenTile + (g + 2) * H : next);
-                __builtin_amdgcn_sched_barrier(0);
-                sub_tile_pairs<T, DIM, PPT, H, X>(B, w, 2, H, m);
-            }
-            tile_done<T, PPT>(tj, m, best, second, wt);
-        }
-    } else if constexpr (X) {
+    if constexpr (X) {
         for (int tj = 0; tj < nT; ++tj) {
             T s[kChosenTile][DIM];
             T b[kChosenTile];
@@ -208,7 +130,7 @@ __device__ __forceinline__ void screen_main(const T (&w)[PPT][DIM], int K, const
 // Sb[d * ss + k]; slots K .. (tile end) hold the far-away sentinel.  Lanes with
 // valid[j] == false get -1.  Every lane of the wave must call this (the near-tie fallback
 // votes with a ballot).
-template <typename T, int DIM, int PPT, bool DB = false>
+template <typename T, int DIM, int PPT>
 __device__ __forceinline__ void nearest_labels(const T (&p)[PPT][DIM], const bool (&valid)[PPT], int K,
                                                const T* __restrict__ Sb, int64_t ss, T rSqr, int (&label)[PPT]) {
 #pragma unroll
@@ -217,7 +139,7 @@ __device__ __forceinline__ void nearest_labels(const T (&p)[PPT][DIM], const boo
     constexpr T big = FpTraits<T>::big;
     T best[PPT], second[PPT];
     int wt[PPT];
-    screen_main<T, DIM, PPT, false, DB>(p, K, Sb, ss, best, second, wt);
+    screen_main<T, DIM, PPT, false>(p, K, Sb, ss, best, second, wt);
 
     // epilogue: winner inside the winning tile, runner-up, exact decision
     const T delta = T(4 * (DIM + 1)) * FpTraits<T>::unit_roundoff;
@@ -427,7 +349,7 @@ __device__ __forceinline__ void x_decide(const T (&p)[PPT][DIM], const T (&q2)[P
 
 // label[j] as nearest_labels; Xb: the expanded table of the segment (row d < DIM: s'_d,
 // row DIM: b), o: the segment's centre, bmax: B above.
-template <typename T, int DIM, int PPT, bool DB = false>
+template <typename T, int DIM, int PPT>
 __device__ __forceinline__ void nearest_labels_x(const T (&p)[PPT][DIM], const bool (&valid)[PPT], int K,
                                                  const T* __restrict__ Sb, const T* __restrict__ Xb, int64_t ss,
                                                  const T (&o)[DIM], T bmax, T rSqr, int (&label)[PPT]) {
@@ -450,7 +372,7 @@ __device__ __forceinline__ void nearest_labels_x(const T (&p)[PPT][DIM], const b
     }
     T best[PPT], second[PPT];
     int wt[PPT];
-    screen_main<T, DIM, PPT, true, DB>(q2, K, Xb, ss, best, second, wt);
+    screen_main<T, DIM, PPT, true>(q2, K, Xb, ss, best, second, wt);
     x_decide<T, DIM, PPT, false>(p, q2, nq, valid, K, Sb, Xb, ss, bmax, rSqr, best, second, wt, label);
 }
 
