@@ -886,7 +886,8 @@ void Engine<T>::autotuneBatchScan() {
         float score[kShapes + 2];
         for (int c = 0; c < kCands; ++c) score[c] = 1e30f;
         // two rounds of one untimed and one timed launch per shape (one round when a launch
-        // takes more than 4 ms: the tuning then costs about 20 scans of the batch)
+        // takes more than 0.2 ms, long enough to time alone: the tuning then costs about 20 scans
+        // of the batch)
         int rounds = 2;
         for (int round = 0; round < rounds; ++round) {
             for (int c = 0; c < kCands; ++c) {
@@ -919,7 +920,7 @@ void Engine<T>::autotuneBatchScan() {
                 float ms = 0;
                 check(hipEventElapsedTime(&ms, e0, e1), "event");
                 score[c] = std::min(score[c], ms);
-                if (round == 0 && c == 0 && ms > 4.0f) rounds = 1;
+                if (round == 0 && c == 0 && ms > 0.2f) rounds = 1;
             }
         }
         float bestMs = 1e30f;
