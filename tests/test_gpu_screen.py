@@ -189,3 +189,20 @@ def test_autotuned_form(oracle):
         assert c.timers()["screen_form"] in (0, 1, 2)
         _say(f"  picked form {c.timers()['screen_form']}")
     np.testing.assert_array_equal(SR, oracle.ridge_detection(P, 0.2, 0.4)[0])
+
+
+def test_option_values():
+    """RDHIP_OPT_SCREEN takes 0..3 and RDHIP_OPT_CHAIN 0..2; other values are refused with a
+    message and leave the context usable."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN, OPT_SCREEN, RDError
+    P = synth.helix3d(5000, seed=3)
+    with Context(3, np.float32) as c:
+        for bad in (-1, 4):
+            with pytest.raises(RDError, match="screen option"):
+                c.set_option(OPT_SCREEN, bad)
+        with pytest.raises(RDError, match="chain option"):
+            c.set_option(OPT_CHAIN, 3)
+        for ok in (0, 1, 2, 3):
+            c.set_option(OPT_SCREEN, ok)
+        c.set_points(P)
+        assert len(c.ridge_detection(0.2, 0.4)) > 0
