@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -499,7 +500,11 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
     curves, noise 0.02, 5% outliers, r1 0.08, r2 0.16, 4 tiles per dimension).  For each
     algorithm: wall time, the phases, and the evolve iterations' algorithmic bytes (every
     tile's own + neighbour samples read once per iteration, chosen points read, summed and
-    updated) over their time, against the 8 TB/s HBM peak."""
+    updated) over their time, against the 8 TB/s HBM peak.  Quality against the noise-free
+    pattern (the reference's assessment_quality measures, on the device): the Hausdorff
+    distance, the median and largest distance from a ridge point to the pattern, the fraction
+    of ridge points within r1 of it (the rest mark outliers), and the median and largest
+    distance from a pattern point to the ridge (how well the ridge covers the pattern)."""
     from ridgedetection_amd import synth
     from ridgedetection_amd._lib import Context, OPT_GRAPH
     out = {}
@@ -507,9 +512,11 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
         if w == "helix":
             P = synth.helix3d(1_000_000, seed=2024, sigma=0.05, dtype=np.float32)
             r1, r2, tpd, cap = 0.2, 0.4, [8, 8, 8], 10000
+            ideal = synth.helix_curve(np.linspace(0.0, 2.0 * math.pi * 5.0, 100_000))
         else:
             P = synth.bezier_union(2_000_000, dim=6, seed=6, dtype=np.float32)
             r1, r2, tpd, cap = 0.08, 0.16, [4] * 6, 10000
+            ideal = synth.bezier_union_curves(dim=6, curves=4, seed=6)
         for alg in algorithms:
             ctx = Context(P.shape[1], np.float32)
             if not graph:
@@ -520,6 +527,11 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
             res = ctx.tiled(alg, r1, r2, cap, tpd)
             wall = time.perf_counter() - t0
             t = ctx.timers()
+            with Context(P.shape[1], np.float64) as q:
+                h = q.hausdorff(res.astype(np.float64), ideal)
+                st = q.distance_stats(res.astype(np.float64), ideal)
+                near = q.set_distances(res.astype(np.float64), ideal) <= r1
+                cov = q.distance_stats(ideal, res.astype(np.float64))
             ctx.close()
             sec = t["evolve_ms"] * 1e-3
             gbs = t["evolve_bytes"] / sec / 1e9 if sec > 0 else 0.0
@@ -527,6 +539,10 @@ def tiled_runs(workloads=("helix", "bezier6"), algorithms=(6, 1, 0), graph=True)
             tflops = pps * 3 * P.shape[1] / 1e12        # the reference's 3d flops per pair
             out[f"{w}/{TILED_NAMES[alg]}"] = {
                 "points": len(P), "dim": P.shape[1], "ridge_points": len(res), "wall_s": wall,
+                "quality": {"hausdorff_to_pattern": h, "median_distance_to_pattern": float(st[0]),
+                            "max_distance_to_pattern": float(st[3]), "r1": r1,
+                            "fraction_within_r1_of_pattern": float(near.mean()) if len(near) else 0.0,
+                            "pattern_to_ridge_median": float(cov[0]), "pattern_to_ridge_max": float(cov[3])},
                 "build_ms": t["build_ms"], "choose_ms": t["choose_ms"], "evolve_ms": t["evolve_ms"],
                 "decimate_ms": t["decimate_ms"], "evolve_iterations": t["evolve_iterations"],
                 "rounds": t["rounds"], "evolve_GBps": gbs, "evolve_frac_of_hbm_peak": gbs / HBM_PEAK_GBS,

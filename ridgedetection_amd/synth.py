@@ -90,6 +90,17 @@ def bezier_union(n: int, dim: int = 6, curves: int = 4, seed: int = 0, sigma: fl
     return np.ascontiguousarray(p.astype(dtype))
 
 
+def bezier_union_curves(dim: int = 6, curves: int = 4, seed: int = 0, per_curve: int = 50000) -> np.ndarray:
+    """Noise-free points of the curves bezier_union(..., dim, curves, seed) samples around (the same
+    control points: they are the generator's first draw), per_curve evenly spaced parameters each."""
+    g = _rng(seed)
+    ctrl = g.uniform(-1.0, 1.0, (curves, 4, dim))
+    u = np.linspace(0.0, 1.0, per_curve)[:, None]
+    pts = [((1 - u) ** 3) * c[0] + 3 * ((1 - u) ** 2) * u * c[1] + 3 * (1 - u) * (u ** 2) * c[2] + (u ** 3) * c[3]
+           for c in ctrl]
+    return np.ascontiguousarray(np.concatenate(pts, axis=0))
+
+
 def segment_chain(n: int, dim: int, seed: int = 0, segments: int = 4, sigma: float = 0.05,
                   dtype=np.float32) -> np.ndarray:
     """Polyline through `segments+1` random vertices in [-1,1]^dim with noise."""
