@@ -110,7 +110,9 @@ def cpu_per_config():
             c.set_points(P)
             t0 = time.perf_counter()
             S = c.ridge_detection(r1, r2, max_evolve_iters=cap)
-            return time.perf_counter() - t0, len(S)
+            el = time.perf_counter() - t0
+            gpu_rd.last = S
+            return el, len(S)
 
     P = synth.spiral2d(20000, seed=1)
     t0 = time.perf_counter()
@@ -127,9 +129,16 @@ def cpu_per_config():
     cpu = time.perf_counter() - t0
     gpu, k = gpu_rd(sub, 0.2, 0.4)
     gfull, kfull = gpu_rd(H, 0.2, 0.4)
+    ideal = synth.helix_curve(np.linspace(0.0, 2.0 * math.pi * 5.0, 100_000))
+    R = gpu_rd.last.astype(np.float64)
+    with Context(3, np.float64) as q:
+        quality = {"hausdorff_to_pattern": q.hausdorff(R, ideal),
+                   "median_distance_to_pattern": float(q.distance_stats(R, ideal)[0]),
+                   "pattern_to_ridge_max": float(q.distance_stats(ideal, R)[3]), "r1": 0.2}
     out["helix_brute"] = {"sample": "first 50 000 of the 1M helix points", "cpu_s": cpu, "gpu_s": gpu,
                           "cpu_ridge_points": len(S), "gpu_ridge_points": k, "gpu_over_cpu_speedup": cpu / gpu,
-                          "gpu_whole_1M_s": gfull, "gpu_whole_1M_ridge_points": kfull}
+                          "gpu_whole_1M_s": gfull, "gpu_whole_1M_ridge_points": kfull,
+                          "gpu_whole_1M_quality": quality}
     log(f"bench: CPU helix 50k whole RD {cpu:.2f} s, GPU {gpu * 1e3:.1f} ms (1M: {gfull:.2f} s)")
     for alg in (0, 5):
         t0 = time.perf_counter()
