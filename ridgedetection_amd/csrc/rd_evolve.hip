@@ -19,9 +19,10 @@
 //      into the running sum in sample order (the reference's serial summation, so sums are
 //      bit-identical), applies the shift rule and raises the segment's "moved" flag.
 //      Chains holding a sorted position that is a multiple of L (every chain of at least L
-//      members; L adapts so that at most kLongWaves waves are needed) go to one wave each
-//      instead, in the same launch, which adds at the latency of a plain dependent add
-//      rather than that of a DPP-fed one.
+//      members; L adapts so that at most kLongWaves entries are needed), or listed by length,
+//      go to a block of four waves each instead, in the same launch: each wave sums a quarter
+//      of the chain from 64 speculative starts at once, one per lane, and the quarter whose
+//      start matches the exact sum before it is kept (chain_sum_parts).
 //   5. evolve_finalize -- per-segment iteration count and convergence; global all-done.
 #include "rd_evolve.hpp"
 #include "rd_rowsum.hpp"
@@ -125,60 +126,78 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Serial sum of src[0 .. cnt) by a group of kLanes lanes (every lane of the wave calls it;
-// `steps` is the wave-uniform largest cnt of its groups); tile is the group's LDS slice.
+// Serial sum start + src[0] + src[1] + ... + src[cnt - 1] by a group of kLanes lanes (every
+// lane of the wave takes part; `steps` is the wave-uniform largest cnt of its groups); tile is
+// the group's LDS slice.  `start` may differ between lanes (each lane then returns its own sum)
+// and must not be -0 (the zeros of the first pass and of the padding leave any other value as
+// it is).  load(0) may be issued early (the first tile's members are then in reg) and the sum
+// run later.
 template <typename T, int kLanes>
-__device__ __forceinline__ T group_ordered_sum(const T* __restrict__ src, int cnt, int steps, T* tile) {
-    using W = typename WideOf<T>::type;
-    constexpr int kW = WideOf<T>::n;
-    constexpr int kTile = kLanes * kLongPer;
-    const int lane = threadIdx.x & (kLanes - 1);
-    const int nTiles = (steps + kTile - 1) / kTile;
-    const int last = max(cnt - 1, 0);
+struct OrderedRun {
+    static constexpr int kTile = kLanes * kLongPer;
+    const T* __restrict__ src;
+    int cnt, last, lane;
     T reg[kLongPer];
-    // index clamped into the chain and masked (tile t + 1 is loaded after tile t is staged;
-    // its loads are not waited for before tile t's sum)
-    auto load = [&](int t) {
+
+    __device__ __forceinline__ OrderedRun(const T* __restrict__ s, int n)
+        : src(s), cnt(n), last(max(n - 1, 0)), lane((int)(threadIdx.x & (kLanes - 1))) {}
+
+    // tile t: index clamped into the chain and masked (tile t + 1 is loaded after tile t is
+    // staged; its loads are not waited for before tile t's sum)
+    __device__ __forceinline__ void load(int t) {
 #pragma unroll
         for (int c = 0; c < kLongPer; ++c) {
             const int i = t * kTile + c * kLanes + lane;
             const T x = src[min(i, last)];
             reg[c] = i < cnt ? x : T(0);
         }
-    };
-    load(0);
-    T sum = T(0);
-    for (int t = 0; t < nTiles; ++t) {
-        wave_lds_sync();
+    }
+
+    // the sum, with tile 0 already in reg
+    __device__ __forceinline__ T run(int steps, T* tile, T start) {
+        using W = typename WideOf<T>::type;
+        constexpr int kW = WideOf<T>::n;
+        const int nTiles = (steps + kTile - 1) / kTile;
+        T sum = start;
+        for (int t = 0; t < nTiles; ++t) {
+            wave_lds_sync();
 #pragma unroll
-        for (int c = 0; c < kLongPer; ++c) tile[c * kLanes + lane] = reg[c];
-        wave_lds_sync();
-        load(t + 1);
-        // groups of kG wide reads: group g + 1 is read while group g is added (the first
-        // pass adds a group of zeros, which leaves the sum unchanged); the scheduling
-        // barriers keep the reads a whole group ahead of their adds
-        const W* w = reinterpret_cast<const W*>(tile);
-        constexpr int kG = 8, kNQ = kTile / kW;
-        static_assert(kNQ % kG == 0, "tile is not a whole number of read groups");
-        W cur[kG];
+            for (int c = 0; c < kLongPer; ++c) tile[c * kLanes + lane] = reg[c];
+            wave_lds_sync();
+            load(t + 1);
+            // groups of kG wide reads: group g + 1 is read while group g is added (the first
+            // pass adds a group of zeros, which leaves the sum unchanged); the scheduling
+            // barriers keep the reads a whole group ahead of their adds
+            const W* w = reinterpret_cast<const W*>(tile);
+            constexpr int kG = 8, kNQ = kTile / kW;
+            static_assert(kNQ % kG == 0, "tile is not a whole number of read groups");
+            W cur[kG];
 #pragma unroll
-        for (int g = 0; g < kG; ++g) cur[g] = W{};
+            for (int g = 0; g < kG; ++g) cur[g] = W{};
 #pragma unroll 2
-        for (int q0 = 0; q0 < kNQ; q0 += kG) {
-            W nxt[kG];
+            for (int q0 = 0; q0 < kNQ; q0 += kG) {
+                W nxt[kG];
 #pragma unroll
-            for (int g = 0; g < kG; ++g) nxt[g] = w[q0 + g];
-            __builtin_amdgcn_sched_barrier(0);
+                for (int g = 0; g < kG; ++g) nxt[g] = w[q0 + g];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < kG; ++g) cur[g] = nxt[g];
+            }
 #pragma unroll
             for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int g = 0; g < kG; ++g) cur[g] = nxt[g];
         }
-#pragma unroll
-        for (int g = 0; g < kG; ++g) sum = add_wide(sum, cur[g]);
+        return sum;
     }
-    return sum;
+};
+
+template <typename T, int kLanes>
+__device__ __forceinline__ T group_ordered_sum(const T* __restrict__ src, int cnt, int steps, T* tile, T start) {
+    OrderedRun<T, kLanes> r(src, cnt);
+    r.load(0);
+    return r.run(steps, tile, start);
 }
 
 // ---------------------------------------------------------------------------------
@@ -282,15 +301,20 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
     const int64_t chain = a.denseChains ? rs : slot;
     int seg = -1, off = 0, cnt = 0;
     if (d < DIM) {
+        // the slot's segment, range and list index are loaded together (in bounds for every slot;
+        // used only once the slot is known to be live)
         seg = a.slotSeg[slot];
+        const int o = a.chainOff[chain];
+        const int ce = a.chainEnd[chain];
+        const int li = a.longCount ? a.longIdx[chain] : 0;
         if (seg >= 0 && !a.segDone[seg] && (int)(slot - a.segs[seg].sOff) < a.segK[seg]) {
-            off = a.chainOff[chain];
-            cnt = a.chainEnd[chain] - off;
+            off = o;
+            cnt = ce - o;
             // a chain holding a multiple of L (or, by length, one of the listed long chains) is
-            // summed by a wave (chain_sum_wave)
+            // summed by a block (chain_sum_parts)
             const int64_t s0 = off, e1 = (int64_t)off + cnt - 1;
             const bool waveSummed =
-                a.longCount ? (cnt >= a.longMin && a.longIdx[chain] < a.longCap)
+                a.longCount ? (cnt >= a.longMin && li < a.longCap)
                             : (a.longList && cnt > 0 &&
                                (e1 >> a.longShift) >= ((s0 + (int64_t(1) << a.longShift) - 1) >> a.longShift));
             if (waveSummed) {
@@ -313,47 +337,153 @@ __device__ __forceinline__ void chain_sum_rows(const ShiftArgs<T>& a, int64_t bl
     }
     const int steps = wave_row_max(cnt);
     const int rowInWave = (threadIdx.x & (kWave - 1)) / kRow;
-    const T sum = group_ordered_sum<T, kRow>(src, cnt, steps, waveLds + rowInWave * (kRowTile + WideOf<T>::n));
+    const T sum = group_ordered_sum<T, kRow>(src, cnt, steps, waveLds + rowInWave * (kRowTile + WideOf<T>::n), T(0));
     if (seg < 0 || t != 0) return;
     shift_coordinate(a, d, slot, seg, cnt, sum);
 }
 
-// wave path: one wave per (list entry, coordinate)
+// Ordered integer of a value (the order of the values, -0 and +0 both 0) and back (never -0).
+__device__ __forceinline__ int64_t ord_of(float x) {
+    const int i = __float_as_int(x);
+    return i >= 0 ? (int64_t)i : -(int64_t)(i & 0x7fffffff);
+}
+__device__ __forceinline__ int64_t ord_of(double x) {
+    const long long i = __double_as_longlong(x);
+    return i >= 0 ? (int64_t)i : -(int64_t)(i & 0x7fffffffffffffffLL);
+}
+template <typename T>
+__device__ __forceinline__ T from_ord(int64_t o);
+template <>
+__device__ __forceinline__ float from_ord<float>(int64_t o) {
+    return __int_as_float(o >= 0 ? (int)o : (int)(0x80000000u | (unsigned)(-o)));
+}
+template <>
+__device__ __forceinline__ double from_ord<double>(int64_t o) {
+    return __longlong_as_double(o >= 0 ? (long long)o : (long long)(0x8000000000000000ULL | (unsigned long long)(-o)));
+}
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+__device__ __forceinline__ bool same_bits(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// Long chains: one block of kParts waves per (list entry, coordinate).  The serial sum of a
+// chain is cut into kParts runs of consecutive members, one per wave, summed at the same time:
+// wave p cannot know the exact partial sum its run starts from (the serial sum of every member
+// before it), so each of its 64 lanes sums the run from its own start, the 64 consecutive values
+// of the type centred on the correctly rounded sum of those members (from the wider sums of the
+// runs before it).  Wave 0 then walks the runs in order: run p's result is the one of the lane
+// whose start equals, bit for bit, the exact sum through run p - 1 -- the same adds of the same
+// values in the same order as one serial sum, so the same result.  When no lane started from that
+// value (the rounding errors of the serial sum before run p exceed 32 units of its last place,
+// about one run in five hundred), wave 0 sums run p again from it.  The result never depends on
+// the estimate; only the time does.
+constexpr int kParts = 256 / kWave;
+
+template <typename T>
+struct PartLds {
+    typename WideSum<T>::Acc wide[kParts];
+    int64_t ordE[kParts];
+    T cand[kParts][kWave];
+    T start[kParts][kWave];
+};
+
 template <typename T, int DIM>
-__device__ __forceinline__ void chain_sum_wave(const ShiftArgs<T>& a, int64_t wave, T* tile) {
-    const int64_t e = wave / DIM;
-    const int d = (int)(wave - e * DIM);
-    if (a.longCount ? e >= min(*a.longCount, a.longCap) : (e << a.longShift) >= *a.mPtr) return;
+__device__ __forceinline__ void chain_sum_parts(const ShiftArgs<T>& a, int64_t entry, T* tile, PartLds<T>& sh) {
+    using W = WideSum<T>;
+    using Run = OrderedRun<T, kWave>;
+    const int64_t e = entry / DIM;
+    const int d = (int)(entry - e * DIM);
+    // every exit before the barriers below is the same for the whole block.  The list entry is
+    // loaded with the live count (in bounds either way: e < the list's capacity), and the chain's
+    // range and segment together once the entry is known to be live.
     const int key = a.longList[e];
+    if (a.longCount ? e >= min(*a.longCount, a.longCap) : (e << a.longShift) >= *a.mPtr) return;
     if (key < 0) return;
     const int slot = a.denseChains ? a.rowSlots[key] : key;
     const int seg = a.slotSeg[slot];
-    if (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg]) return;
     const int off = a.chainOff[key];
     const int cnt = a.chainEnd[key] - off;
+    // a chain listed by length belongs to a live segment and a live slot (the member scatter
+    // lists no other); a chain listed by position is checked
+    if (!a.longCount && (seg < 0 || a.segDone[seg] || (int)(slot - a.segs[seg].sOff) >= a.segK[seg])) return;
+    const T* __restrict__ base = a.sortedP + (int64_t)d * a.mStride + off;
+    const int w = threadIdx.x / kWave, lane = lane_id();
     if (a.tolerance) {
-        const double wide = WideSum<T>::value(group_parallel_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt));
-        if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate_wide(a, d, slot, seg, cnt, wide);
+        if (w != 0) return;
+        const double wide = W::value(group_parallel_sum<T, kWave>(base, cnt));
+        if (lane == 0) shift_coordinate_wide(a, d, slot, seg, cnt, wide);
         return;
     }
-    const T sum = group_ordered_sum<T, kWave>(a.sortedP + (int64_t)d * a.mStride + off, cnt, cnt, tile);
-    if ((threadIdx.x & (kWave - 1)) == 0) shift_coordinate(a, d, slot, seg, cnt, sum);
+    auto run_of = [&](int p, int& n) {
+        const int b0 = (int)((int64_t)cnt * p / kParts);
+        n = (int)((int64_t)cnt * (p + 1) / kParts) - b0;
+        return base + (n > 0 ? b0 : 0);   // an empty run reads (and discards) the chain's first member
+    };
+    int n;
+    const T* __restrict__ src = run_of(w, n);
+    // the run's first tile, then the wider sum of the whole run from those registers and the
+    // loads of the rest, issued before any of them is waited for
+    Run run(src, n);
+    run.load(0);
+    typename W::Acc acc[4] = {W::zero(), W::zero(), W::zero(), W::zero()};
+    for (int i0 = Run::kTile; i0 < n; i0 += Run::kTile) {
+        T v[kLongPer];
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) {
+            const int i = i0 + c * kWave + lane;
+            v[c] = i < n ? src[i] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < kLongPer; ++c) acc[c & 3] = W::add(acc[c & 3], v[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < kLongPer; ++c) acc[c & 3] = W::add(acc[c & 3], run.reg[c]);
+    typename W::Acc wide = W::add(W::add(acc[0], acc[1]), W::add(acc[2], acc[3]));
+#pragma unroll
+    for (int m = kWave / 2; m >= 1; m /= 2) wide = W::add(wide, W::xorShfl(wide, m));
+    if (lane == 0) sh.wide[w] = wide;
+    __syncthreads();
+    T start = T(0);
+    if (w > 0) {
+        typename W::Acc est = W::zero();
+        for (int q = 0; q < w; ++q) est = W::add(est, sh.wide[q]);
+        const T e0 = (T)W::value(est);
+        const int64_t oe = ord_of(e0);
+        start = isfinite(e0) ? from_ord<T>(oe + lane - kWave / 2) : T(0);
+        if (lane == 0) sh.ordE[w] = oe;
+    }
+    const T cand = run.run(n, tile, start);
+    sh.cand[w][lane] = cand;
+    sh.start[w][lane] = start;
+    __syncthreads();
+    if (w != 0) return;
+    T x = sh.cand[0][0];
+    for (int p = 1; p < kParts; ++p) {
+        const int64_t o = ord_of(x) - sh.ordE[p] + kWave / 2;
+        if (o >= 0 && o < kWave && same_bits(sh.start[p][o], x)) {
+            x = sh.cand[p][o];
+        } else {
+            int np;
+            const T* __restrict__ sp = run_of(p, np);
+            x = group_ordered_sum<T, kWave>(sp, np, np, tile, x);
+        }
+    }
+    if (lane == 0) shift_coordinate(a, d, slot, seg, cnt, x);
 }
 
-// blocks [0, rowBlocks): 16-lane rows; the rest: four waves each on listed chains.  With
+// blocks [0, rowBlocks): 16-lane rows; the rest: one listed chain and coordinate each.  With
 // a.fin set, the last block to finish (a device-scope completion count, released by a fence
 // in every block) runs the iteration's convergence bookkeeping, saving its launch.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
     __shared__ __align__(16) T tiles[256 / kWave][wave_lds_elems<T>()];
+    __shared__ PartLds<T> parts;
     __shared__ int lastBlock;
     if (*a.allDone) return;
-    if ((int64_t)blockIdx.x < rowBlocks) {
+    if ((int64_t)blockIdx.x < rowBlocks)
         chain_sum_rows<T, DIM>(a, blockIdx.x, tiles[threadIdx.x / kWave]);
-    } else {
-        const int w = threadIdx.x / kWave;
-        chain_sum_wave<T, DIM>(a, ((int64_t)blockIdx.x - rowBlocks) * (256 / kWave) + w, tiles[w]);
-    }
+    else
+        chain_sum_parts<T, DIM>(a, (int64_t)blockIdx.x - rowBlocks, tiles[threadIdx.x / kWave], parts);
     if (!a.fin.enabled) return;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -403,7 +533,7 @@ hipError_t launch_chain_sum_shift(const ShiftArgs<T>& a, int dim, int64_t maxM, 
     if (rowBlocks <= 0) return hipSuccess;
     const int64_t waves = a.longCount ? (int64_t)dim * a.longCap
                                       : a.longList ? (int64_t)dim * long_list_entries(maxM, a.longShift) : 0;
-    const int64_t blocks = rowBlocks + (waves + 256 / kWave - 1) / (256 / kWave);
+    const int64_t blocks = rowBlocks + waves;
     const unsigned gblocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxM + 255) / 256, 8192));
     bool ok = false;
     RD_DISPATCH_DIM(dim, {
