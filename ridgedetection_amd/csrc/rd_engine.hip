@@ -701,12 +701,11 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
         ha.longIdx = chainLongIdx_.get();
         ha.longMin = 1 << kLongMinShift;
         ha.longCap = kLongWaves / dim_;
-        const bool fuseFinalize = finalize && segs_.size() == 1;
-        if (fuseFinalize)
-            ha.fin = {1, 1, dSegDone_.get(), dSegIter_.get(), capIter, flags_.get(), flags_.get() + 1,
-                      reinterpret_cast<unsigned*>(flags_.get() + 8)};
+        // the convergence bookkeeping in a launch of its own, not in the sum launch's last
+        // block: that form's device-scope fence and completion count in each of the launch's
+        // ~2000 blocks cost ~100 us per iteration of a 322k-point cloud against ~5 us here
         check(launch_chain_sum_shift<T>(ha, dim_, chainMStride_, false, st_), "chain_sum_shift");
-        if (finalize && !fuseFinalize)
+        if (finalize)
             check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(),
                                          capIter, flags_.get(), flags_.get() + 1, st_),
                   "finalize");
@@ -746,16 +745,9 @@ void Engine<T>::enqueueIteration(bool record, int slot, const IterLog* log, bool
         ha.rowSlots = rowSlots_.get();
         ha.denseChains = dense ? 1 : 0;
     }
-    // A single cloud: the sum launch's last block does the convergence bookkeeping (one
-    // launch fewer).  Segment sets keep the separate launch: the device-scope fence every
-    // block of the fused form issues writes back the L2's dirty lines, which after the
-    // member scatter of a large batch costs more than the launch.
-    const bool fuseFinalize = finalize && segs_.size() == 1;
-    if (fuseFinalize)
-        ha.fin = {1, (int)segs_.size(), dSegDone_.get(), dSegIter_.get(), capIter, flags_.get(), flags_.get() + 1,
-                  reinterpret_cast<unsigned*>(flags_.get() + 8)};
+    // the convergence bookkeeping in a launch of its own (as above)
     check(launch_chain_sum_shift<T>(ha, dim_, totalP, false, st_), "chain_sum_shift");
-    if (finalize && !fuseFinalize)
+    if (finalize)
         check(launch_evolve_finalize((int)segs_.size(), dSegDone_.get(), dSegMoved_.get(), dSegIter_.get(), capIter,
                                      flags_.get(), flags_.get() + 1, st_),
               "finalize");
@@ -1261,8 +1253,6 @@ long long Engine<T>::evolve(long long maxIter, const std::vector<char>& active) 
     check(hipMemcpyAsync(dSegDone_.get(), done.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "done");
     check(hipMemcpyAsync(dSegMoved_.get(), zeros.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st_), "moved");
     check(hipMemcpyAsync(flags_.get(), f, sizeof(int) * 2, hipMemcpyHostToDevice, st_), "flags");
-    const unsigned zeroBlocks = 0;   // completion count of the fused bookkeeping
-    check(hipMemcpyAsync(flags_.get() + 8, &zeroBlocks, sizeof(unsigned), hipMemcpyHostToDevice, st_), "flags");
     autotuneScan();
     segmentedNow_ = chooseSegmented(done);
     if (segmentedNow_) {

@@ -471,31 +471,17 @@ __device__ __forceinline__ void chain_sum_parts(const ShiftArgs<T>& a, int64_t e
     if (lane == 0) shift_coordinate(a, d, slot, seg, cnt, x);
 }
 
-// blocks [0, rowBlocks): 16-lane rows; the rest: one listed chain and coordinate each.  With
-// a.fin set, the last block to finish (a device-scope completion count, released by a fence
-// in every block) runs the iteration's convergence bookkeeping, saving its launch.
+// blocks [0, rowBlocks): 16-lane rows; the rest: one listed chain and coordinate each.  The
+// convergence bookkeeping follows in a launch of its own (evolve_finalize_kernel).
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void chain_sum_shift_kernel(ShiftArgs<T> a, int64_t rowBlocks) {
     __shared__ __align__(16) T tiles[256 / kWave][wave_lds_elems<T>()];
     __shared__ PartLds<T> parts;
-    __shared__ int lastBlock;
     if (*a.allDone) return;
     if ((int64_t)blockIdx.x < rowBlocks)
         chain_sum_rows<T, DIM>(a, blockIdx.x, tiles[threadIdx.x / kWave]);
     else
         chain_sum_parts<T, DIM>(a, (int64_t)blockIdx.x - rowBlocks, tiles[threadIdx.x / kWave], parts);
-    if (!a.fin.enabled) return;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        lastBlock = atomicAdd(a.fin.blocksDone, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!lastBlock) return;
-    __threadfence();
-    finalize_segments(a.fin.numSegs, a.fin.segDone, a.segMoved, a.fin.segIter, a.fin.maxIter, a.fin.allDone,
-                      a.fin.totalIters);
-    if (threadIdx.x == 0) *a.fin.blocksDone = 0;
 }
 
 // ---------------------------------------------------------------------------------

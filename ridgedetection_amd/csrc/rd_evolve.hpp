@@ -92,18 +92,6 @@ struct ShiftArgs {
     // optional: the slots the rows visit (row r: coordinate r / totalSlots, slot
     // rowSlots[r % totalSlots]); null: every slot below totalSlots
     const int* rowSlots;
-    // optional: the iteration's convergence bookkeeping (evolve_finalize) run by the last
-    // block of the sum launch; blocksDone is 0 before the launch and again after it
-    struct Finalize {
-        int enabled;
-        int numSegs;
-        int* segDone;
-        int* segIter;
-        int maxIter;
-        int* allDone;
-        int* totalIters;
-        unsigned* blocksDone;
-    } fin;
     // chains are indexed by the dense key (row r's chain is r, a listed chain's slot is
     // rowSlots[key]) instead of the slot
     int denseChains;

@@ -4,7 +4,9 @@ correctly rounded sum before it), and a run whose exact start is not among them 
 from it. These cases force that second pass: one chain's members are a, -a, b, -b (four runs of
 equal length), so the exact sum before the third run is 0 while the serial sum holds only its
 rounding residue, far outside any window around 0. Its other coordinate and a second chain of
-positive members take the first pass. Bit-exact against the oracle for every member ordering
+positive members take the first pass. A chain of 40 000 members covers runs that start after
+long prefixes (10 000 to 30 000 members), where the serial sum strays furthest from the
+correctly rounded one. Bit-exact against the oracle for every member ordering
 (RDHIP_OPT_CHAIN 0, 1, 2), FP32 and FP64."""
 
 import numpy as np
@@ -56,3 +58,27 @@ def test_second_pass(oracle, dtype, chain):
             assert flag == oflag
             S = S1
     assert ocounts.min() >= 1024
+
+
+@pytest.mark.parametrize("chain", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_long_prefix_windows(oracle, dtype, chain):
+    """One chain of 40 000 members: runs of 10 000 members after prefixes of up to 30 000."""
+    from ridgedetection_amd._lib import Context, OPT_CHAIN
+    rng = np.random.default_rng(62)
+    P = rng.uniform(4.0, 6.0, (40000, 2)).astype(dtype)
+    S = np.array([[5.0, 5.0]], dtype=dtype)
+    _say(f"case long prefix {np.dtype(dtype).name} chain option {chain}")
+    with Context(2, dtype) as c:
+        c.set_option(OPT_CHAIN, chain)
+        c.set_points(P)
+        c.set_chosen(S)
+        for _ in range(2):
+            oS, oflag, ocounts, osums, olabels, omoved = oracle.evolve_iter(P, S, 2.0)
+            S1, flag, counts, sums, labels, moved = c.evolve_iteration(2.0)
+            np.testing.assert_array_equal(counts, ocounts)
+            np.testing.assert_array_equal(sums, osums)
+            np.testing.assert_array_equal(S1, oS)
+            assert flag == oflag
+            S = S1
+    assert ocounts[0] == 40000
