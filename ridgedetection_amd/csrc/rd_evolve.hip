@@ -409,9 +409,15 @@ __device__ __forceinline__ void chain_sum_parts(const ShiftArgs<T>& a, int64_t e
     const T* __restrict__ base = a.sortedP + (int64_t)d * a.mStride + off;
     const int w = threadIdx.x / kWave, lane = lane_id();
     if (a.tolerance) {
+        // a quarter of the chain per wave in the wider type, combined in a fixed order
+        const int q0 = (int)((int64_t)cnt * w / kParts), q1 = (int)((int64_t)cnt * (w + 1) / kParts);
+        const typename W::Acc part = group_parallel_sum<T, kWave>(base + (q1 > q0 ? q0 : 0), q1 - q0);
+        if (lane == 0) sh.wide[w] = part;
+        __syncthreads();
         if (w != 0) return;
-        const double wide = W::value(group_parallel_sum<T, kWave>(base, cnt));
-        if (lane == 0) shift_coordinate_wide(a, d, slot, seg, cnt, wide);
+        typename W::Acc total = W::zero();
+        for (int q = 0; q < kParts; ++q) total = W::add(total, sh.wide[q]);
+        if (lane == 0) shift_coordinate_wide(a, d, slot, seg, cnt, W::value(total));
         return;
     }
     auto run_of = [&](int p, int& n) {
